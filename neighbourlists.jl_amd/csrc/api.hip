@@ -1,0 +1,889 @@
+// C ABI of libnlhip.so: handles, device buffers, host geometry and the
+// orchestration of the kernel pipeline (kernels.hip).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/nlhip.h"
+#include "../../include/nlhip_device.h"
+#include "nlhip_internal.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+thread_local int g_device = 0;
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) {
+        const int code = (e == hipErrorOutOfMemory) ? NLH_ENOMEM : NLH_EDEVICE;
+        throw Error(code, std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+#define HIPC(x) hip_check((x), #x)
+
+template <typename F>
+int guarded(F &&f)
+{
+    try {
+        f();
+        return NLH_OK;
+    } catch (const Error &e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host allocation failed";
+        return NLH_ENOMEM;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return NLH_EDEVICE;
+    }
+}
+
+// Grow-only device buffer.
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    template <typename T>
+    T *as() const { return static_cast<T *>(p); }
+    // returns true when (re)allocated
+    bool ensure(size_t bytes, bool zero = false)
+    {
+        if (bytes == 0) bytes = 16;
+        if (bytes <= cap) return false;
+        // growth is rare: wait for every stream before the old block goes
+        HIPC(hipDeviceSynchronize());
+        if (p) HIPC(hipFree(p));
+        p = nullptr;
+        const size_t nb = std::max(bytes + bytes / 8, (size_t)256);
+        HIPC(hipMalloc(&p, nb));
+        cap = nb;
+        if (zero) {
+            HIPC(hipMemset(p, 0, nb));
+            HIPC(hipDeviceSynchronize());
+        }
+        return true;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+}  // namespace
+
+struct nlh_handle {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevBuf X, sys, atom_base, cell_base, key, slot, cell_count, cell_start, tmp, rec, wbig, perm, cell_id,
+        cell_offsets, pair_count, pair_off, partials, sys_pair_base, first, i, j, S, R, status, Xsorted, nb_j, nb_R,
+        nb_S, nb_count;
+    nlh::Status *status_host = nullptr;   // pinned
+    void *staging = nullptr;              // pinned upload staging for system tables
+    size_t staging_cap = 0;
+    // state of the last build
+    int nsys = 0;
+    int64_t nat = 0;
+    int32_t M = 0;
+    int ti_bytes = 4;
+    bool built = false, materialized = false, have_R = false;
+    int64_t npairs = 0;
+    int64_t pair_capacity = 0;
+    const double *X_dev = nullptr;
+    std::vector<nlh::SysDev> sys_host;
+    std::vector<int64_t> atom_base_host;
+    std::vector<int32_t> cell_base_host;
+    std::vector<nlh_geometry> geoms;
+    std::vector<int64_t> pair_offsets_host;
+    // per-phase timing (nlh_set_profiling): events on the handle's stream
+    bool profile = false;
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    float phase_ms[5] = {0, 0, 0, 0, 0};
+    bool ev_build = false, ev_pairs = false;
+    // captured batch pipeline
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    bool graph_want_R = false;
+
+    nlh::PipelineBufs bufs() const
+    {
+        nlh::PipelineBufs b{};
+        b.X = X_dev;
+        b.sys = sys.as<nlh::SysDev>();
+        b.sys_atom_base = atom_base.as<int64_t>();
+        b.sys_cell_base = cell_base.as<int32_t>();
+        b.nsys = nsys;
+        b.nat = nat;
+        b.M = M;
+        b.key = key.as<int32_t>();
+        b.slot = slot.as<int32_t>();
+        b.cell_count = cell_count.as<int32_t>();
+        b.cell_start = cell_start.as<int32_t>();
+        b.tmp = tmp.as<int32_t>();
+        b.rec = rec.as<nlh::Rec>();
+        b.wbig = wbig.as<int32_t>();
+        b.perm_out = perm.p;
+        b.cell_id_out = cell_id.p;
+        b.cell_offsets_out = cell_offsets.p;
+        b.pair_count = pair_count.as<int32_t>();
+        b.pair_off = pair_off.as<int64_t>();
+        b.scan_partials = partials.as<int64_t>();
+        b.sys_pair_base = sys_pair_base.as<int64_t>();
+        b.first_out = first.p;
+        b.i_out = i.p;
+        b.j_out = j.p;
+        b.S_out = S.p;
+        b.R_out = R.as<double>();
+        b.pair_capacity = pair_capacity;
+        b.status = status.as<nlh::Status>();
+        b.ti_bytes = ti_bytes;
+        return b;
+    }
+};
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// host geometry: analyze_cell (src/cell_list.jl:152-170) with StaticArrays'
+// 3x3 evaluation order; this file is compiled with -ffp-contract=off.
+// ---------------------------------------------------------------------------
+
+void cross3(const double a[3], const double b[3], double o[3])
+{
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+double dot3(const double a[3], const double b[3]) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+
+void inv3(const double a[9], double out[9])
+{
+    double x0[3] = {a[0], a[1], a[2]};
+    const double x1[3] = {a[3], a[4], a[5]}, x2[3] = {a[6], a[7], a[8]};
+    double y0[3], y1[3], y2[3];
+    cross3(x1, x2, y0);
+    const double d = dot3(x0, y0);
+    for (int k = 0; k < 3; ++k) {
+        x0[k] = x0[k] / d;
+        y0[k] = y0[k] / d;
+    }
+    cross3(x2, x0, y1);
+    cross3(x0, x1, y2);
+    for (int c = 0; c < 3; ++c) {
+        out[0 + 3 * c] = y0[c];
+        out[1 + 3 * c] = y1[c];
+        out[2 + 3 * c] = y2[c];
+    }
+}
+
+double det3(const double a[9])
+{
+    const double x0[3] = {a[0], a[1], a[2]}, x1[3] = {a[3], a[4], a[5]}, x2[3] = {a[6], a[7], a[8]};
+    double y[3];
+    cross3(x1, x2, y);
+    return dot3(x0, y);
+}
+
+// lengths (src/cell_list.jl:94-95): rows of C are the lattice vectors
+void lengths3(const double c[9], double out[3])
+{
+    double r[3][3];
+    for (int m = 0; m < 3; ++m)
+        for (int k = 0; k < 3; ++k) r[m][k] = c[m + 3 * k];
+    const double d = det3(c);
+    double t[3];
+    auto nrm = [](const double v[3]) { return std::sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]); };
+    cross3(r[1], r[2], t);
+    out[0] = d / nrm(t);
+    cross3(r[2], r[0], t);
+    out[1] = d / nrm(t);
+    cross3(r[0], r[1], t);
+    out[2] = d / nrm(t);
+}
+
+// Neighbour-cell reach limits: (2 nxyz + 1)^3 cells are visited per atom.
+constexpr int64_t kMaxNxyz = 64;
+constexpr int64_t kMaxSegments = (int64_t)1 << 21;
+
+void validate_geometry(const nlh_geometry &g)
+{
+    if (g.ti_bytes != 4 && g.ti_bytes != 8) throw Error(NLH_EINVAL, "ti_bytes must be 4 (Int32) or 8 (Int64)");
+    if (!(g.cutoff > 0.0) || !std::isfinite(g.cutoff)) throw Error(NLH_EINVAL, "cutoff must be positive and finite");
+    for (int k = 0; k < 3; ++k) {
+        if (g.ncells[k] < 1) throw Error(NLH_EINVAL, "ncells must be >= 1");
+        if (g.nxyz[k] < 0 || g.nxyz[k] > kMaxNxyz)
+            throw Error(NLH_EINVAL, "neighbour-cell reach (nxyz) out of the supported range [0, 64]");
+    }
+    if ((2 * g.nxyz[0] + 1) * (2 * g.nxyz[1] + 1) * (2 * g.nxyz[2] + 1) > kMaxSegments)
+        throw Error(NLH_EINVAL, "too many neighbour cells per atom (cutoff far larger than the cell)");
+    if (g.ncells_total != g.ncells[0] * g.ncells[1] * g.ncells[2]) throw Error(NLH_EINVAL, "ncells_total mismatch");
+    const int64_t tmax = g.ti_bytes == 4 ? INT32_MAX : INT64_MAX;
+    if (g.ncells_total > tmax)
+        throw Error(NLH_EOVERFLOW, "Ratio of simulation cell size to cutoff is very large. Use a larger integer "
+                                   "type (e.g. Int64), larger cutoff, or smaller simulation cell.");
+    if (g.ncells_total > INT32_MAX - 1)
+        throw Error(NLH_EOVERFLOW, "more than 2^31-1 cells per build is not supported on the device");
+}
+
+void geometry_init(const double cell[9], double cutoff, const uint8_t pbc[3], int32_t ti_bytes, nlh_geometry &g)
+{
+    std::memset(&g, 0, sizeof g);
+    if (ti_bytes != 4 && ti_bytes != 8) throw Error(NLH_EINVAL, "ti_bytes must be 4 (Int32) or 8 (Int64)");
+    if (!(cutoff > 0.0) || !std::isfinite(cutoff)) throw Error(NLH_EINVAL, "cutoff must be positive and finite");
+    for (int q = 0; q < 9; ++q)
+        if (!std::isfinite(cell[q])) throw Error(NLH_EINVAL, "cell has non-finite entries");
+    std::memcpy(g.cell, cell, sizeof g.cell);
+    g.cutoff = cutoff;
+    g.cutoff_sq = cutoff * cutoff;   // cutoff^2 (gpu_kernels.jl:317)
+    g.ti_bytes = ti_bytes;
+    for (int k = 0; k < 3; ++k) g.pbc[k] = pbc[k] ? 1 : 0;
+    g.zero_volume = std::fabs(det3(cell)) < 1e-12;
+    inv3(cell, g.inv_cell);
+    double lens[3];
+    lengths3(cell, lens);
+    const double timax = ti_bytes == 4 ? 2147483647.0 : 9.2233720368547758e18;
+    for (int k = 0; k < 3; ++k) {
+        g.lens[k] = std::fabs(lens[k]);
+        const double t = std::floor(g.lens[k] / cutoff);   // floor.(TI, lens / cutoff)
+        if (std::isnan(t)) throw Error(NLH_EINVAL, "degenerate cell (face height is NaN)");
+        if (t > timax) throw Error(NLH_EOVERFLOW, "ncells does not fit the index type");
+        const int64_t n = (int64_t)t;
+        g.ncells[k] = n < 1 ? 1 : n;
+    }
+    const long double prod = (long double)g.ncells[0] * g.ncells[1] * g.ncells[2];
+    const long double tmax = ti_bytes == 4 ? 2147483647.0L : 9223372036854775807.0L;
+    if (prod > tmax)
+        throw Error(NLH_EOVERFLOW, "Ratio of simulation cell size to cutoff is very large. Use a larger integer "
+                                   "type (e.g. Int64), larger cutoff, or smaller simulation cell.");
+    g.ncells_total = g.ncells[0] * g.ncells[1] * g.ncells[2];
+    for (int k = 0; k < 3; ++k) {
+        // ceil.(TI, cutoff * (ncells ./ abs.(lens)))  (gpu_kernels.jl:315-316)
+        const double v = std::ceil(cutoff * ((double)g.ncells[k] / g.lens[k]));
+        if (!std::isfinite(v) || v > timax) throw Error(NLH_EINVAL, "neighbour-cell reach is not finite (zero-volume cell?)");
+        g.nxyz[k] = (int64_t)v;
+    }
+}
+
+nlh::SysDev make_sys(const nlh_geometry &g, int64_t nat, int64_t atom_base, int32_t cell_base, int64_t first_base)
+{
+    nlh::SysDev s{};
+    std::memcpy(s.C, g.cell, sizeof s.C);
+    std::memcpy(s.inv, g.inv_cell, sizeof s.inv);
+    s.rc2 = g.cutoff_sq;
+    for (int k = 0; k < 3; ++k) {
+        s.n[k] = (int32_t)g.ncells[k];
+        s.nxyz[k] = (int32_t)g.nxyz[k];
+    }
+    s.pbc = (g.pbc[0] ? 1 : 0) | (g.pbc[1] ? 2 : 0) | (g.pbc[2] ? 4 : 0);
+    s.nat = (int32_t)nat;
+    s.atom_base = atom_base;
+    s.cell_base = cell_base;
+    s.M = (int32_t)g.ncells_total;
+    s.first_base = first_base;
+    return s;
+}
+
+void ensure_staging(nlh_handle *h, size_t bytes)
+{
+    if (bytes <= h->staging_cap) return;
+    HIPC(hipStreamSynchronize(h->stream));
+    if (h->staging) HIPC(hipHostFree(h->staging));
+    h->staging = nullptr;
+    const size_t nb = std::max(bytes * 2, (size_t)4096);
+    HIPC(hipHostMalloc(&h->staging, nb, hipHostMallocDefault));
+    h->staging_cap = nb;
+}
+
+void drop_graph(nlh_handle *h)
+{
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    h->graph_exec = nullptr;
+    h->graph = nullptr;
+}
+
+// Set up the system tables of a batch (a single build is nsys == 1) and
+// size every build-side buffer.  Uploads go through pinned staging.
+void setup_batch(nlh_handle *h, int32_t nsys, const int64_t *atom_offsets, const nlh_geometry *geoms)
+{
+    if (nsys < 1) throw Error(NLH_EINVAL, "nsys must be >= 1");
+    const int ti = geoms[0].ti_bytes;
+    h->sys_host.resize(nsys);
+    h->atom_base_host.resize(nsys + 1);
+    h->cell_base_host.resize(nsys + 1);
+    h->geoms.assign(geoms, geoms + nsys);
+    int64_t cells = 0;
+    if (atom_offsets[0] != 0) throw Error(NLH_EINVAL, "atom_offsets[0] must be 0");
+    for (int s = 0; s < nsys; ++s) {
+        validate_geometry(geoms[s]);
+        if (geoms[s].ti_bytes != ti) throw Error(NLH_EINVAL, "all systems of a batch must share ti_bytes");
+        const int64_t na = atom_offsets[s + 1] - atom_offsets[s];
+        if (na < 0) throw Error(NLH_EINVAL, "atom_offsets must be non-decreasing");
+        if (ti == 4 && na >= INT32_MAX) throw Error(NLH_EOVERFLOW, "too many atoms for Int32 indices");
+        h->sys_host[s] = make_sys(geoms[s], na, atom_offsets[s], (int32_t)cells, atom_offsets[s] + s);
+        h->atom_base_host[s] = atom_offsets[s];
+        h->cell_base_host[s] = (int32_t)cells;
+        cells += geoms[s].ncells_total;
+        if (cells > INT32_MAX - 1) throw Error(NLH_EOVERFLOW, "more than 2^31-1 cells in one batch");
+    }
+    const int64_t nat = atom_offsets[nsys];
+    if (nat > INT32_MAX - 1) throw Error(NLH_EOVERFLOW, "more than 2^31-1 atoms in one build");
+    h->atom_base_host[nsys] = nat;
+    h->cell_base_host[nsys] = (int32_t)cells;
+    h->nsys = nsys;
+    h->nat = nat;
+    h->M = (int32_t)cells;
+    h->ti_bytes = ti;
+
+    const size_t n1 = (size_t)std::max<int64_t>(nat, 1);
+    const size_t M1 = (size_t)cells + 1;
+    h->sys.ensure(nsys * sizeof(nlh::SysDev));
+    h->atom_base.ensure((nsys + 1) * sizeof(int64_t));
+    h->cell_base.ensure((nsys + 1) * sizeof(int32_t));
+    h->key.ensure(n1 * 4);
+    h->slot.ensure(n1 * 4);
+    h->cell_count.ensure(M1 * 4, /*zero=*/true);
+    h->cell_start.ensure(M1 * 4);
+    h->tmp.ensure(n1 * 4);
+    h->rec.ensure(n1 * sizeof(nlh::Rec));
+    h->wbig.ensure(n1 * 12);
+    h->perm.ensure(n1 * ti);
+    h->cell_id.ensure(n1 * ti);
+    h->cell_offsets.ensure((M1 + nsys) * ti);
+    h->partials.ensure((std::max(M1, n1) / nlh::kScanTile + 2) * sizeof(int64_t));
+    h->status.ensure(sizeof(nlh::Status));
+
+    const size_t sys_bytes = nsys * sizeof(nlh::SysDev);
+    const size_t ab_bytes = (nsys + 1) * sizeof(int64_t);
+    const size_t cb_bytes = (nsys + 1) * sizeof(int32_t);
+    ensure_staging(h, sys_bytes + ab_bytes + cb_bytes + 64);
+    char *st = static_cast<char *>(h->staging);
+    std::memcpy(st, h->sys_host.data(), sys_bytes);
+    std::memcpy(st + sys_bytes, h->atom_base_host.data(), ab_bytes);
+    std::memcpy(st + sys_bytes + ab_bytes, h->cell_base_host.data(), cb_bytes);
+    HIPC(hipMemcpyAsync(h->sys.p, st, sys_bytes, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(h->atom_base.p, st + sys_bytes, ab_bytes, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(h->cell_base.p, st + sys_bytes + ab_bytes, cb_bytes, hipMemcpyHostToDevice, h->stream));
+}
+
+void upload_positions(nlh_handle *h, const double *X, uint32_t flags)
+{
+    if (h->nat > 0 && X == nullptr) throw Error(NLH_EINVAL, "X is NULL");
+    if (flags & NLH_X_DEVICE) {
+        h->X_dev = X;
+    } else {
+        h->X.ensure((size_t)std::max<int64_t>(h->nat, 1) * 24);
+        if (h->nat > 0) HIPC(hipMemcpyAsync(h->X.p, X, (size_t)h->nat * 24, hipMemcpyHostToDevice, h->stream));
+        h->X_dev = h->X.as<double>();
+    }
+}
+
+void reset_status(nlh_handle *h)
+{
+    // Status is cleared by a tiny copy from pinned memory (no memset node).
+    std::memset(h->status_host, 0, sizeof(nlh::Status));
+    HIPC(hipMemcpyAsync(h->status.p, h->status_host, sizeof(nlh::Status), hipMemcpyHostToDevice, h->stream));
+}
+
+void mark(nlh_handle *h, int k)
+{
+    if (h->profile) HIPC(hipEventRecord(h->ev[k], h->stream));
+}
+
+void build_cells(nlh_handle *h)
+{
+    mark(h, 0);
+    nlh::launch_build(h->bufs(), h->stream);
+    mark(h, 1);
+    h->ev_build = h->profile;
+    h->ev_pairs = false;
+    HIPC(hipGetLastError());
+    h->built = true;
+    h->materialized = false;
+}
+
+int64_t estimate_pairs(const nlh_handle *h)
+{
+    // rough density estimate; a wrong guess costs one extra scan + fill
+    long double est = 0;
+    for (int s = 0; s < h->nsys; ++s) {
+        const nlh_geometry &g = h->geoms[s];
+        const double vol = std::fabs(det3(g.cell));
+        const double nat = (double)h->sys_host[s].nat;
+        if (nat <= 0) continue;
+        const double rho = vol > 0 ? nat / vol : 0.0;
+        const double ball = 4.18879020478639 * g.cutoff * g.cutoff * g.cutoff;
+        est += (long double)nat * (rho * ball * 1.25 + 4.0);
+    }
+    return (int64_t)std::min<long double>(est, 4.0e11L) + 1024;
+}
+
+void ensure_pair_outputs(nlh_handle *h, int64_t cap, bool want_R)
+{
+    const int ti = h->ti_bytes;
+    h->i.ensure((size_t)cap * ti);
+    h->j.ensure((size_t)cap * ti);
+    h->S.ensure((size_t)cap * 3 * ti);
+    if (want_R) h->R.ensure((size_t)cap * 24);
+    int64_t c = (int64_t)(h->i.cap / ti);
+    c = std::min<int64_t>(c, (int64_t)(h->j.cap / ti));
+    c = std::min<int64_t>(c, (int64_t)(h->S.cap / (3 * ti)));
+    if (want_R) c = std::min<int64_t>(c, (int64_t)(h->R.cap / 24));
+    h->pair_capacity = c;
+}
+
+void ensure_pair_buffers(nlh_handle *h)
+{
+    const size_t n1 = (size_t)std::max<int64_t>(h->nat, 1);
+    h->pair_count.ensure(n1 * 4);
+    h->pair_off.ensure((n1 + 1) * 8);
+    h->first.ensure((n1 + h->nsys) * h->ti_bytes);
+    h->sys_pair_base.ensure((h->nsys + 1) * 8);
+}
+
+void finish_status(nlh_handle *h)
+{
+    HIPC(hipMemcpyAsync(h->status_host, h->status.p, sizeof(nlh::Status), hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+}
+
+void check_positions(nlh_handle *h)
+{
+    if (h->status_host->bad_position) {
+        h->built = false;
+        throw Error(NLH_EINVAL, "positions must be finite and within 1e9 cells of the box");
+    }
+}
+
+// Pair stages.  Returns after the wait, with npairs known.
+void materialize(nlh_handle *h, uint32_t flags)
+{
+    if (!h->built) throw Error(NLH_ESTATE, "materialize called before a successful build_cell_list");
+    const bool want_R = (flags & NLH_WANT_R) != 0;
+    ensure_pair_buffers(h);
+    if (h->nat == 0) {   // gpu_kernels.jl:303-312: first = [1]
+        finish_status(h);
+        check_positions(h);
+        h->npairs = 0;
+        if (h->nsys == 1) {
+            const int64_t one = 1;
+            if (h->ti_bytes == 8) HIPC(hipMemcpy(h->first.p, &one, 8, hipMemcpyHostToDevice));
+            else {
+                const int32_t o = 1;
+                HIPC(hipMemcpy(h->first.p, &o, 4, hipMemcpyHostToDevice));
+            }
+        }
+        h->materialized = true;
+        h->have_R = want_R;
+        h->pair_offsets_host.assign(h->nsys + 1, 0);
+        return;
+    }
+    ensure_pair_outputs(h, std::max(h->pair_capacity, estimate_pairs(h)), want_R);
+    nlh::PipelineBufs b = h->bufs();
+    mark(h, 1);
+    nlh::launch_count(b, h->stream);
+    mark(h, 2);
+    nlh::launch_scan_pairs(b, h->stream);
+    mark(h, 3);
+    nlh::launch_fill(b, want_R, h->stream);
+    mark(h, 4);
+    h->ev_pairs = h->profile;
+    HIPC(hipGetLastError());
+    finish_status(h);
+    check_positions(h);
+    if (h->status_host->overflow) {   // grow and redo the scan + fill
+        ensure_pair_outputs(h, h->status_host->npairs, want_R);
+        b = h->bufs();
+        nlh::launch_scan_pairs(b, h->stream);
+        nlh::launch_fill(b, want_R, h->stream);
+        HIPC(hipGetLastError());
+        finish_status(h);
+        if (h->status_host->overflow) throw Error(NLH_EDEVICE, "pair capacity still exceeded after growth");
+    }
+    h->npairs = h->status_host->npairs;
+    h->have_R = want_R;
+    // per-system pair offsets
+    h->pair_offsets_host.resize(h->nsys + 1);
+    if (h->nsys == 1) {
+        h->pair_offsets_host[0] = 0;
+        h->pair_offsets_host[1] = h->npairs;
+    } else {
+        HIPC(hipMemcpy(h->pair_offsets_host.data(), h->sys_pair_base.p, (h->nsys + 1) * 8, hipMemcpyDeviceToHost));
+    }
+    if (h->ti_bytes == 4)
+        for (int s = 0; s < h->nsys; ++s)
+            if (h->pair_offsets_host[s + 1] - h->pair_offsets_host[s] >= INT32_MAX)
+                throw Error(NLH_EOVERFLOW, "pair count exceeds typemax(Int32); use int_type=Int64");
+    h->materialized = true;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// neighbours of one atom (lazy API), via the device-side header
+// ---------------------------------------------------------------------------
+
+template <typename TI>
+__global__ void neighbours_one_kernel(nlh_celllist_dev<TI> c, int64_t i, int64_t cap, TI *j, double *R, TI *S,
+                                      int64_t *count)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    int64_t n = 0;
+    nlh_for_each_neighbour(c, i, [&](TI jj, const double *RR, const TI *SS) {
+        if (n < cap) {
+            j[n] = jj;
+            for (int k = 0; k < 3; ++k) {
+                R[3 * n + k] = RR[k];
+                S[3 * n + k] = SS[k];
+            }
+        }
+        ++n;
+    });
+    *count = n;
+}
+
+extern "C" {
+
+int nlh_version(int32_t *major, int32_t *minor)
+{
+    if (major) *major = 0;
+    if (minor) *minor = 1;
+    return NLH_OK;
+}
+
+const char *nlh_last_error(void) { return g_last_error.c_str(); }
+
+int nlh_init(int device)
+{
+    return guarded([&] {
+        int n = 0;
+        HIPC(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n) throw Error(NLH_EINVAL, "no such device");
+        HIPC(hipSetDevice(device));
+        g_device = device;
+    });
+}
+
+int nlh_create(nlh_handle **out)
+{
+    return guarded([&] {
+        if (!out) throw Error(NLH_EINVAL, "out is NULL");
+        int n = 0;
+        HIPC(hipGetDeviceCount(&n));
+        if (n == 0) throw Error(NLH_EDEVICE, "no HIP device");
+        auto *h = new nlh_handle();
+        h->device = g_device;
+        try {
+            HIPC(hipSetDevice(h->device));
+            HIPC(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+            HIPC(hipHostMalloc((void **)&h->status_host, sizeof(nlh::Status), hipHostMallocDefault));
+            std::memset(h->status_host, 0, sizeof(nlh::Status));
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int nlh_destroy(nlh_handle *h)
+{
+    if (!h) return NLH_OK;
+    return guarded([&] {
+        (void)hipSetDevice(h->device);
+        if (h->stream) (void)hipStreamSynchronize(h->stream);
+        drop_graph(h);
+        for (hipEvent_t &e : h->ev)
+            if (e) (void)hipEventDestroy(e);
+        DevBuf *bufs[] = {&h->X,        &h->sys,       &h->atom_base,  &h->cell_base,    &h->key,    &h->slot,
+                          &h->cell_count, &h->cell_start, &h->tmp,      &h->rec,          &h->wbig,   &h->perm,
+                          &h->cell_id,  &h->cell_offsets, &h->pair_count, &h->pair_off,   &h->partials,
+                          &h->sys_pair_base, &h->first, &h->i,          &h->j,            &h->S,      &h->R,
+                          &h->status,   &h->Xsorted,   &h->nb_j,       &h->nb_R,         &h->nb_S,   &h->nb_count};
+        for (DevBuf *b : bufs) b->release();
+        if (h->status_host) (void)hipHostFree(h->status_host);
+        if (h->staging) (void)hipHostFree(h->staging);
+        if (h->stream) (void)hipStreamDestroy(h->stream);
+        delete h;
+    });
+}
+
+int nlh_synchronize(nlh_handle *h)
+{
+    return guarded([&] {
+        if (!h) throw Error(NLH_EINVAL, "handle is NULL");
+        HIPC(hipSetDevice(h->device));
+        finish_status(h);
+        check_positions(h);
+    });
+}
+
+int nlh_get_stream(nlh_handle *h, void **stream)
+{
+    return guarded([&] {
+        if (!h || !stream) throw Error(NLH_EINVAL, "NULL argument");
+        *stream = (void *)h->stream;
+    });
+}
+
+int nlh_geometry_init(const double cell[9], double cutoff, const uint8_t pbc[3], int32_t ti_bytes, nlh_geometry *out)
+{
+    return guarded([&] {
+        if (!cell || !pbc || !out) throw Error(NLH_EINVAL, "NULL argument");
+        geometry_init(cell, cutoff, pbc, ti_bytes, *out);
+    });
+}
+
+int nlh_build_cell_list(nlh_handle *h, const double *X, int64_t nat, const nlh_geometry *geom, uint32_t flags)
+{
+    return guarded([&] {
+        if (!h || !geom) throw Error(NLH_EINVAL, "NULL argument");
+        if (nat < 0) throw Error(NLH_EINVAL, "nat < 0");
+        HIPC(hipSetDevice(h->device));
+        h->built = false;
+        drop_graph(h);
+        const int64_t offs[2] = {0, nat};
+        setup_batch(h, 1, offs, geom);
+        upload_positions(h, X, flags);
+        reset_status(h);
+        build_cells(h);
+        if (!(flags & NLH_NO_SYNC)) {
+            finish_status(h);
+            check_positions(h);
+        }
+    });
+}
+
+int nlh_materialize(nlh_handle *h, uint32_t flags, int64_t *npairs)
+{
+    return guarded([&] {
+        if (!h) throw Error(NLH_EINVAL, "handle is NULL");
+        HIPC(hipSetDevice(h->device));
+        materialize(h, flags);
+        if (npairs) *npairs = h->npairs;
+    });
+}
+
+int nlh_neighbour_list(nlh_handle *h, const double *X, int64_t nat, const nlh_geometry *geom, uint32_t flags,
+                       int64_t *npairs)
+{
+    return guarded([&] {
+        if (!h || !geom) throw Error(NLH_EINVAL, "NULL argument");
+        if (nat < 0) throw Error(NLH_EINVAL, "nat < 0");
+        HIPC(hipSetDevice(h->device));
+        h->built = false;
+        drop_graph(h);
+        const int64_t offs[2] = {0, nat};
+        setup_batch(h, 1, offs, geom);
+        upload_positions(h, X, flags);
+        reset_status(h);
+        build_cells(h);
+        materialize(h, flags);
+        if (npairs) *npairs = h->npairs;
+    });
+}
+
+int nlh_pairlist_device(nlh_handle *h, nlh_pairlist_view *out)
+{
+    return guarded([&] {
+        if (!h || !out) throw Error(NLH_EINVAL, "NULL argument");
+        if (!h->materialized) throw Error(NLH_ESTATE, "no materialised pair list");
+        out->nat = h->nat;
+        out->npairs = h->npairs;
+        out->i = h->i.p;
+        out->j = h->j.p;
+        out->S = h->S.p;
+        out->first = h->first.p;
+        out->R = h->have_R ? h->R.as<double>() : nullptr;
+        out->ti_bytes = h->ti_bytes;
+    });
+}
+
+int nlh_celllist_device(nlh_handle *h, nlh_celllist_view *out)
+{
+    return guarded([&] {
+        if (!h || !out) throw Error(NLH_EINVAL, "NULL argument");
+        if (!h->built) throw Error(NLH_ESTATE, "no cell list built");
+        if (h->nsys != 1) throw Error(NLH_ESTATE, "cell-list views are per single-system build");
+        out->nat = h->nat;
+        out->ncells_total = h->M;
+        out->X_orig = h->X_dev;
+        out->perm = h->perm.p;
+        out->cell_id = h->cell_id.p;
+        out->cell_offsets = h->cell_offsets.p;
+        out->ti_bytes = h->ti_bytes;
+    });
+}
+
+int nlh_copy_pairlist(nlh_handle *h, void *i, void *j, void *S, void *first, double *R)
+{
+    return guarded([&] {
+        if (!h) throw Error(NLH_EINVAL, "handle is NULL");
+        if (!h->materialized) throw Error(NLH_ESTATE, "no materialised pair list");
+        HIPC(hipSetDevice(h->device));
+        HIPC(hipStreamSynchronize(h->stream));
+        const size_t P = (size_t)h->npairs, ti = (size_t)h->ti_bytes;
+        if (P > 0) {
+            if (i) HIPC(hipMemcpy(i, h->i.p, P * ti, hipMemcpyDefault));
+            if (j) HIPC(hipMemcpy(j, h->j.p, P * ti, hipMemcpyDefault));
+            if (S) HIPC(hipMemcpy(S, h->S.p, 3 * P * ti, hipMemcpyDefault));
+            if (R) {
+                if (!h->have_R) throw Error(NLH_ESTATE, "R was not requested (NLH_WANT_R)");
+                HIPC(hipMemcpy(R, h->R.p, 3 * P * 8, hipMemcpyDefault));
+            }
+        }
+        if (first) HIPC(hipMemcpy(first, h->first.p, ((size_t)h->nat + h->nsys) * ti, hipMemcpyDefault));
+    });
+}
+
+int nlh_copy_celllist(nlh_handle *h, double *X_sorted, void *perm, void *cell_id, void *cell_offsets)
+{
+    return guarded([&] {
+        if (!h) throw Error(NLH_EINVAL, "handle is NULL");
+        if (!h->built) throw Error(NLH_ESTATE, "no cell list built");
+        HIPC(hipSetDevice(h->device));
+        const size_t n = (size_t)h->nat, ti = (size_t)h->ti_bytes;
+        if (X_sorted && n > 0) {
+            h->Xsorted.ensure(n * 24);
+            nlh::launch_sorted_positions(h->bufs(), h->Xsorted.as<double>(), h->stream);
+            HIPC(hipGetLastError());
+        }
+        HIPC(hipStreamSynchronize(h->stream));
+        if (X_sorted && n > 0) HIPC(hipMemcpy(X_sorted, h->Xsorted.p, n * 24, hipMemcpyDefault));
+        if (perm && n > 0) HIPC(hipMemcpy(perm, h->perm.p, n * ti, hipMemcpyDefault));
+        if (cell_id && n > 0) HIPC(hipMemcpy(cell_id, h->cell_id.p, n * ti, hipMemcpyDefault));
+        if (cell_offsets)
+            HIPC(hipMemcpy(cell_offsets, h->cell_offsets.p, ((size_t)h->M + h->nsys) * ti, hipMemcpyDefault));
+    });
+}
+
+int nlh_neighbours(nlh_handle *h, int32_t sys, int64_t i, int64_t cap, void *j, double *R, void *S, int64_t *count)
+{
+    return guarded([&] {
+        if (!h) throw Error(NLH_EINVAL, "handle is NULL");
+        if (!h->built) throw Error(NLH_ESTATE, "no cell list built");
+        if (sys < 0 || sys >= h->nsys) throw Error(NLH_EINVAL, "no such system");
+        const nlh::SysDev &sd = h->sys_host[sys];
+        if (i < 1 || i > sd.nat) throw Error(NLH_EINVAL, "atom index out of range (1-based)");
+        if (cap < 0) throw Error(NLH_EINVAL, "cap < 0");
+        HIPC(hipSetDevice(h->device));
+        const int ti = h->ti_bytes;
+        const size_t c1 = (size_t)std::max<int64_t>(cap, 1);
+        h->nb_j.ensure(c1 * ti);
+        h->nb_R.ensure(c1 * 24);
+        h->nb_S.ensure(c1 * 3 * ti);
+        h->nb_count.ensure(8);
+        nlh_celllist_view v{};
+        v.nat = sd.nat;
+        v.ncells_total = sd.M;
+        v.X_orig = h->X_dev + 3 * sd.atom_base;
+        v.perm = static_cast<char *>(h->perm.p) + (size_t)sd.atom_base * ti;
+        v.cell_offsets = static_cast<char *>(h->cell_offsets.p) + ((size_t)sd.cell_base + sys) * ti;
+        v.ti_bytes = ti;
+        if (ti == 8) {
+            auto c = nlh_make_celllist_dev<int64_t>(v, h->geoms[sys]);
+            hipLaunchKernelGGL(neighbours_one_kernel<int64_t>, dim3(1), dim3(64), 0, h->stream, c, i, cap,
+                               h->nb_j.as<int64_t>(), h->nb_R.as<double>(), h->nb_S.as<int64_t>(),
+                               h->nb_count.as<int64_t>());
+        } else {
+            auto c = nlh_make_celllist_dev<int32_t>(v, h->geoms[sys]);
+            hipLaunchKernelGGL(neighbours_one_kernel<int32_t>, dim3(1), dim3(64), 0, h->stream, c, i, cap,
+                               h->nb_j.as<int32_t>(), h->nb_R.as<double>(), h->nb_S.as<int32_t>(),
+                               h->nb_count.as<int64_t>());
+        }
+        HIPC(hipGetLastError());
+        HIPC(hipStreamSynchronize(h->stream));
+        int64_t n = 0;
+        HIPC(hipMemcpy(&n, h->nb_count.p, 8, hipMemcpyDeviceToHost));
+        const size_t m = (size_t)std::min(n, cap);
+        if (m > 0) {
+            if (j) HIPC(hipMemcpy(j, h->nb_j.p, m * ti, hipMemcpyDeviceToHost));
+            if (R) HIPC(hipMemcpy(R, h->nb_R.p, m * 24, hipMemcpyDeviceToHost));
+            if (S) HIPC(hipMemcpy(S, h->nb_S.p, m * 3 * ti, hipMemcpyDeviceToHost));
+        }
+        if (count) *count = n;
+    });
+}
+
+int nlh_set_profiling(nlh_handle *h, int enable)
+{
+    return guarded([&] {
+        if (!h) throw Error(NLH_EINVAL, "handle is NULL");
+        HIPC(hipSetDevice(h->device));
+        if (enable && !h->ev[0])
+            for (hipEvent_t &e : h->ev) HIPC(hipEventCreate(&e));
+        h->profile = enable != 0;
+    });
+}
+
+int nlh_phase_times(nlh_handle *h, float ms[5])
+{
+    return guarded([&] {
+        if (!h || !ms) throw Error(NLH_EINVAL, "NULL argument");
+        HIPC(hipSetDevice(h->device));
+        for (int k = 0; k < 5; ++k) ms[k] = 0.0f;
+        if (h->ev_build) HIPC(hipEventElapsedTime(&ms[0], h->ev[0], h->ev[1]));
+        if (h->ev_pairs) {
+            HIPC(hipEventElapsedTime(&ms[1], h->ev[1], h->ev[2]));
+            HIPC(hipEventElapsedTime(&ms[2], h->ev[2], h->ev[3]));
+            HIPC(hipEventElapsedTime(&ms[3], h->ev[3], h->ev[4]));
+        }
+        if (h->ev_build && h->ev_pairs) HIPC(hipEventElapsedTime(&ms[4], h->ev[0], h->ev[4]));
+    });
+}
+
+int nlh_batch_neighbour_list(nlh_handle *h, int32_t nsys, const double *X_batch, const int64_t *atom_offsets,
+                             const nlh_geometry *geoms, uint32_t flags, int64_t *total_pairs)
+{
+    return guarded([&] {
+        if (!h || !atom_offsets || !geoms) throw Error(NLH_EINVAL, "NULL argument");
+        HIPC(hipSetDevice(h->device));
+        h->built = false;
+        drop_graph(h);
+        setup_batch(h, nsys, atom_offsets, geoms);
+        upload_positions(h, X_batch, flags);
+        reset_status(h);
+        build_cells(h);
+        materialize(h, flags);
+        if (total_pairs) *total_pairs = h->npairs;
+    });
+}
+
+int nlh_batch_pair_offsets(nlh_handle *h, int64_t *pair_offsets)
+{
+    return guarded([&] {
+        if (!h || !pair_offsets) throw Error(NLH_EINVAL, "NULL argument");
+        if (!h->materialized) throw Error(NLH_ESTATE, "no materialised pair list");
+        std::memcpy(pair_offsets, h->pair_offsets_host.data(), (h->nsys + 1) * sizeof(int64_t));
+    });
+}
+
+int nlh_batch_rebuild(nlh_handle *h, const double *X_batch, uint32_t flags, int64_t *total_pairs)
+{
+    return guarded([&] {
+        if (!h) throw Error(NLH_EINVAL, "handle is NULL");
+        if (h->sys_host.empty()) throw Error(NLH_ESTATE, "no previous batch to rebuild");
+        HIPC(hipSetDevice(h->device));
+        upload_positions(h, X_batch, flags);
+        reset_status(h);
+        build_cells(h);
+        materialize(h, flags);
+        if (total_pairs) *total_pairs = h->npairs;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,183 @@
+// Internal structures shared by the nlhip kernels and the host orchestration.
+//
+// Conventions (NeighbourLists.jl, src/types.jl:34-82):
+//   - cell / inv_cell are 3x3 column-major (Julia SMatrix memory order),
+//     element (m,k) at [m + 3k]; ROWS of cell are lattice vectors.
+//   - indices handed back to the caller are 1-based; internal indices are
+//     0-based.
+//   - floating point follows the reference's non-fused left-to-right order;
+//     every HIP source is compiled with -ffp-contract=off.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nlh {
+
+constexpr int kWave = 64;
+constexpr int kPairsBlock = 256;          // 4 waves, one home cell per wave
+constexpr int kWavesPerPairsBlock = kPairsBlock / kWave;
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;            // elements per thread in a scan tile
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+// One system of a batch (a single build is a batch of one).
+struct SysDev {
+    double C[9];        // cell, column-major
+    double inv[9];      // inv(cell), column-major
+    double rc2;         // cutoff^2
+    int32_t n[3];       // ncells per axis
+    int32_t nxyz[3];    // neighbour-cell reach per axis
+    int32_t pbc;        // bit k set = periodic along lattice vector k
+    int32_t nat;        // atoms in this system
+    int64_t atom_base;  // first global atom index
+    int32_t cell_base;  // first global cell index
+    int32_t M;          // cells in this system
+    int64_t first_base; // offset of this system's `first` array (atom_base + sys index)
+};
+
+// Sorted atom record: position, original (system-local, 0-based) index and
+// the packed winding numbers.  32 bytes, one dwordx4 pair per lane.
+struct __attribute__((aligned(16))) Rec {
+    double x, y, z;
+    int32_t j;
+    uint32_t w;         // 3 x 10-bit biased windings; bit 31 = stored in wbig
+};
+
+constexpr uint32_t kWindBig = 0x80000000u;
+
+__host__ __device__ inline uint32_t pack_wind(int32_t wx, int32_t wy, int32_t wz)
+{
+    if (wx < -512 || wx > 511 || wy < -512 || wy > 511 || wz < -512 || wz > 511) return kWindBig;
+    return (uint32_t)(wx + 512) | ((uint32_t)(wy + 512) << 10) | ((uint32_t)(wz + 512) << 20);
+}
+
+__host__ __device__ inline void unpack_wind(uint32_t w, int32_t &wx, int32_t &wy, int32_t &wz)
+{
+    wx = (int32_t)(w & 1023u) - 512;
+    wy = (int32_t)((w >> 10) & 1023u) - 512;
+    wz = (int32_t)((w >> 20) & 1023u) - 512;
+}
+
+// Device-side status block, copied to pinned host memory once per build.
+struct Status {
+    int64_t npairs;          // total pairs over the batch
+    int64_t max_sys_pairs;   // largest per-system pair count (TI overflow check)
+    int32_t overflow;        // pair capacity exceeded: outputs not written
+    int32_t bad_position;    // non-finite / out-of-range position seen
+    int32_t pad[2];
+};
+
+// ---------------------------------------------------------------------------
+// Cell-index algebra (src/cell_list.jl:66-147)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int32_t floordiv_pos(int32_t a, int32_t b)
+{
+    int32_t q = a / b;
+    return (a % b != 0 && a < 0) ? q - 1 : q;
+}
+
+// wrap_and_shift (cell_list.jl:105-120): wrapped index in [1,n] and shift,
+// or clamp with zero shift on an open axis.
+__device__ __forceinline__ void wrap_and_shift(int32_t i, int32_t n, bool p, int32_t &w, int32_t &s)
+{
+    if (!p) {
+        w = i < 1 ? 1 : (i > n ? n : i);
+        s = 0;
+        return;
+    }
+    if (i >= 1 && i <= n) {
+        w = i;
+        s = 0;
+        return;
+    }
+    int32_t q = floordiv_pos(i - 1, n);
+    w = i - q * n;
+    s = q;
+}
+
+// position_to_cell_index (cell_list.jl:66-74): frac = inv_cell' * x,
+// c_k = trunc(floor(frac_k * n_k + 1)).  Returns false when the value does
+// not fit a 32-bit cell index (the reference's unsafe_trunc is undefined
+// there).
+__device__ __forceinline__ bool cell_index(const SysDev &s, double x, double y, double z, int32_t c[3])
+{
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double f = (s.inv[0 + 3 * k] * x + s.inv[1 + 3 * k] * y) + s.inv[2 + 3 * k] * z;
+        const double v = floor(f * (double)s.n[k] + 1.0);
+        ok = ok && (v > -1.0e9 && v < 1.0e9);   // also false for NaN
+        c[k] = ok ? (int32_t)v : 1;
+    }
+    return ok;
+}
+
+// position_to_cell_index + bin_wrap_and_shift (cell_list.jl:141-147)
+__device__ __forceinline__ bool cell_and_winding(const SysDev &s, double x, double y, double z,
+                                                 int32_t c[3], int32_t w[3])
+{
+    int32_t c0[3];
+    const bool ok = cell_index(s, x, y, z, c0);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) wrap_and_shift(c0[k], s.n[k], (s.pbc >> k) & 1, c[k], w[k]);
+    return ok;
+}
+
+// Largest s in [0, n) with base[s] <= x (base ascending, base[0] == 0).
+template <typename T>
+__device__ __forceinline__ int find_sys(const T *__restrict__ base, int n, T x)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (base[mid] <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Host-side launchers (kernels.hip)
+struct PipelineBufs {
+    // inputs
+    const double *X;            // 3 * nat_total, AoS
+    const SysDev *sys;          // nsys
+    const int64_t *sys_atom_base;   // nsys + 1
+    const int32_t *sys_cell_base;   // nsys + 1
+    int32_t nsys;
+    int64_t nat;                // total atoms
+    int32_t M;                  // total cells
+    // build
+    int32_t *key;               // nat: global cell id
+    int32_t *slot;              // nat: arrival rank from the histogram atomics
+    int32_t *cell_count;        // M + 1, zero between builds
+    int32_t *cell_start;        // M + 1
+    int32_t *tmp;               // nat: cell-grouped, not yet stable
+    Rec *rec;                   // nat: sorted records
+    int32_t *wbig;              // 3 * nat: windings that do not fit the pack
+    void *perm_out;             // nat, TI, 1-based
+    void *cell_id_out;          // nat, TI, 1-based
+    void *cell_offsets_out;     // M + nsys, TI, 1-based per system
+    // pairs
+    int32_t *pair_count;        // nat
+    int64_t *pair_off;          // nat + 1
+    int64_t *scan_partials;     // scan block sums
+    int64_t *sys_pair_base;     // nsys + 1 (device)
+    void *first_out;            // nat + nsys, TI, 1-based per system
+    void *i_out, *j_out, *S_out;
+    double *R_out;
+    int64_t pair_capacity;
+    Status *status;
+    int ti_bytes;
+};
+
+void launch_build(const PipelineBufs &b, hipStream_t st);
+void launch_count(const PipelineBufs &b, hipStream_t st);
+void launch_scan_pairs(const PipelineBufs &b, hipStream_t st);
+void launch_fill(const PipelineBufs &b, bool want_R, hipStream_t st);
+void launch_iexpand(const PipelineBufs &b, hipStream_t st);
+void launch_sorted_positions(const PipelineBufs &b, double *Xs, hipStream_t st);
+void launch_neighbours_one(const PipelineBufs &b, int sys, int64_t atom, int64_t cap, void *j,
+                           double *R, void *S, int64_t *count, hipStream_t st);
+
+}  // namespace nlh

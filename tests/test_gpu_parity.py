@@ -1,0 +1,164 @@
+"""GPU parity: libnlhip vs the CPU oracle on the same seeded inputs.
+
+Bar (integer/index work): bit-exact.  Because the device emits pairs in the
+reference traversal order, the raw arrays i, j, S, first (and R, computed in
+the reference's evaluation order) must equal the oracle's element for
+element -- a stronger check than the reference's own set comparison
+(test/test_utils.jl:127-131).
+"""
+import numpy as np
+import pytest
+
+import nlhip
+import oracle
+from nlhip import workloads
+
+pytestmark = pytest.mark.gpu
+
+ALL_PBC = [(True, True, True), (False, False, False), (True, False, False), (False, True, False),
+           (False, False, True), (True, True, False), (True, False, True), (False, True, True)]
+
+
+def _same(nl, ref, R=True):
+    assert nl.npairs() == ref.npairs
+    assert np.array_equal(nl.first, ref.first)
+    assert np.array_equal(nl.i, ref.i)
+    assert np.array_equal(nl.j, ref.j)
+    assert np.array_equal(nl.S, ref.S)
+    if R:
+        assert np.array_equal(np.asarray(nl.R).view(np.int64), ref.R.view(np.int64))
+
+
+def _check(X, rc, C, pbc, int_type=np.int32):
+    ti = 32 if np.dtype(int_type) == np.int32 else 64
+    nl = nlhip.neighbour_list(X, rc, C, pbc, int_type=int_type, want_R=True)
+    ref = oracle.neighbour_list(X, rc, C, pbc, ti_bits=ti, want_R=True)
+    assert nl.i.dtype == np.dtype(int_type)
+    _same(nl, ref)
+    return nl, ref
+
+
+def test_edge_cases():
+    # test/test_utils.jl:480-494
+    C = np.eye(3) * 10.0
+    assert nlhip.neighbour_list(np.array([[5.0, 5, 5]]), 3.0, C, (True,) * 3).npairs() == 0
+    nl = nlhip.neighbour_list(np.array([[5.0, 5, 5], [5, 5, 6]]), 3.0, C, (True,) * 3)
+    assert nl.npairs() == 2
+    assert nlhip.max_neighbours(nl) == 1
+    assert nlhip.neighbour_list(np.array([[1.0, 1, 1], [8, 8, 8]]), 3.0, C, (False,) * 3).npairs() == 0
+    empty = nlhip.neighbour_list(np.zeros((0, 3)), 3.0, C, (True,) * 3)
+    assert empty.npairs() == 0 and list(empty.first) == [1]
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4, 5])
+def test_random_configs(seed):
+    N = 50 + (seed * 37) % 151
+    s = workloads.random_cubic(N, seed=seed)
+    L = s.cell[0, 0]
+    print(f"random N={N} seed={seed}", flush=True)
+    _check(s.X, L / 3, s.cell, (True, True, True))
+
+
+@pytest.mark.parametrize("pbc", ALL_PBC)
+def test_all_pbc(pbc):
+    s = workloads.random_cubic(50, seed=21)
+    L = s.cell[0, 0]
+    print(f"pbc={pbc}", flush=True)
+    _check(s.X, 0.33 * L, s.cell, pbc)
+
+
+@pytest.mark.parametrize("pbc", ALL_PBC)
+def test_triclinic(pbc):
+    C = np.array([[10.0, 2.0, 1.0], [0.0, 9.0, 1.5], [0.0, 0.0, 8.0]])   # test_utils.jl:297-299
+    s = workloads.random_in_cell(80, C, seed=5)
+    print(f"triclinic pbc={pbc}", flush=True)
+    _check(s.X, 3.0, C, pbc)
+
+
+def test_elongated_and_large_cutoff():
+    C = np.diag([5.0, 5.0, 20.0])                                       # test_sortbased.jl:35-40
+    s = workloads.random_in_cell(80, C, seed=6)
+    _check(s.X, 3.0, C, (True, True, True))
+    s = workloads.random_cubic(30, seed=8)                              # test_sortbased.jl:49-55
+    _check(s.X, 0.6 * s.cell[0, 0], s.cell, (True, True, True))
+
+
+def test_high_density_multi_image():
+    # test_gpu.jl:78-86: L=5, 200 atoms, cutoff 2
+    X = 5.0 * workloads.uniform(9, 600).reshape(200, 3)
+    _check(X, 2.0, np.eye(3) * 5.0, (True, True, True))
+    # cells far smaller than the cutoff: nxyz > 1 on every axis
+    X = workloads.uniform(10, 12).reshape(4, 3) @ np.diag([2.2, 3.0, 2.5])
+    nl, ref = _check(X, 6.0, np.diag([2.2, 3.0, 2.5]), (True, True, True))
+    assert nl.npairs() > 100
+
+
+@pytest.mark.parametrize("int_type", [np.int32, np.int64])
+def test_int_types(int_type):
+    s = workloads.random_cubic(100, seed=12)
+    _check(s.X, s.cell[0, 0] / 3, s.cell, (True, True, True), int_type)
+
+
+def test_c0_1000_atoms():
+    s = workloads.random_cubic(1000, density=0.05, seed=100, cutoff=5.0)
+    nl, ref = _check(s.X, 5.0, s.cell, s.pbc)
+    assert 24000 < nl.npairs() < 28500
+
+
+def test_atoms_outside_box():
+    # test_sortbased.jl:143-228 (issue #6): per-atom +-L shifts on periodic
+    # axes, and the frac-just-below-zero rounding trap
+    for pbc in ALL_PBC:
+        s = workloads.random_cubic(60, seed=31)
+        L = s.cell[0, 0]
+        idx = np.arange(1, 61)
+        sh = np.stack([(idx % 3) - 1, ((idx + 1) % 3) - 1, ((idx + 2) % 3) - 1], 1).astype(np.float64)
+        Xp = s.X + L * sh * np.array(pbc, dtype=np.float64)[None, :]
+        _check(Xp, 0.25 * L, s.cell, pbc)
+    L = 8.0
+    tiny = -5e-17 * L
+    X = np.array([[0.5, L - 0.3, 0.5], [0.5, tiny, 0.5]])
+    nl, _ = _check(X, 1.5, np.eye(3) * L, (True, True, True))
+    J, R, S = nlhip.neighbours(nl, 1)
+    assert 2 in list(J)
+    k = list(J).index(2)
+    assert abs(np.sqrt((R[k] ** 2).sum()) - 0.3) < 1e-9
+
+
+def test_far_windings():
+    # windings beyond the packed 10-bit range take the wide path
+    s = workloads.random_cubic(64, seed=41)
+    L = s.cell[0, 0]
+    Xp = s.X.copy()
+    Xp[::3, 0] += 1000.0 * L
+    Xp[1::3, 1] -= 777.0 * L
+    _check(Xp, 0.3 * L, s.cell, (True, True, True))
+
+
+def test_sorted_cell_list_fields():
+    s = workloads.random_cubic(500, seed=51)
+    cl = nlhip.build_cell_list(s.X, 5.0, s.cell, s.pbc)
+    ref = oracle.neighbour_list(s.X, 5.0, s.cell, s.pbc)
+    assert np.array_equal(cl.perm, ref.perm)
+    assert np.array_equal(cl.cell_id, ref.cell_id)
+    assert np.array_equal(cl.cell_offsets, ref.cell_offsets)
+    assert np.array_equal(cl.X, s.X[ref.perm - 1])
+    # lazy per-atom queries equal the materialised list (cell_list.jl:779-833)
+    nl = nlhip.materialize_pairlist(cl, want_R=True)
+    for i in (1, 2, 77, 500):
+        js, Rs, Ss = nlhip.neighbours(cl, i)
+        a, b = ref.first[i - 1] - 1, ref.first[i] - 1
+        assert np.array_equal(js, ref.j[a:b])
+        assert np.array_equal(Ss, ref.S[a:b])
+        assert np.array_equal(Rs.view(np.int64), ref.R[a:b].view(np.int64))
+        assert nlhip.count_neighbours(cl, i) == b - a
+        assert nlhip.num_neighbours(nl, i) == b - a
+
+
+def test_batch_matches_single():
+    systems = workloads.batch_systems(64, seed=3, nmax=400)
+    res = nlhip.batch_neighbour_list(systems, want_R=True)
+    for s, sysm in enumerate(systems):
+        ref = oracle.neighbour_list(sysm.X, sysm.cutoff, sysm.cell, sysm.pbc, want_R=True)
+        pl = res.pairlist(s)
+        _same(pl, ref)
