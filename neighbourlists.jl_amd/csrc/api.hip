@@ -89,8 +89,8 @@ struct DevBuf {
 struct nlh_handle {
     int device = 0;
     hipStream_t stream = nullptr;
-    DevBuf X, sys, atom_base, cell_base, key, slot, cell_count, cell_start, tmp, rec, wbig, perm, cell_id,
-        cell_offsets, pair_count, pair_off, partials, sys_pair_base, first, i, j, S, R, status, Xsorted, nb_j, nb_R,
+    DevBuf X, sys, atom_base, cell_base, brick_base, key, slot, cell_count, cell_start, tmp, rec, wbig, perm, cell_id,
+        cell_offsets, pair_count, masks, pair_off, partials, sys_pair_base, first, i, j, S, R, status, Xsorted, nb_j, nb_R,
         nb_S, nb_count;
     nlh::Status *status_host = nullptr;   // pinned
     void *staging = nullptr;              // pinned upload staging for system tables
@@ -107,6 +107,8 @@ struct nlh_handle {
     std::vector<nlh::SysDev> sys_host;
     std::vector<int64_t> atom_base_host;
     std::vector<int32_t> cell_base_host;
+    std::vector<int32_t> brick_base_host;
+    int64_t nbricks = 0;
     std::vector<nlh_geometry> geoms;
     std::vector<int64_t> pair_offsets_host;
     // per-phase timing (nlh_set_profiling): events on the handle's stream
@@ -126,6 +128,8 @@ struct nlh_handle {
         b.sys = sys.as<nlh::SysDev>();
         b.sys_atom_base = atom_base.as<int64_t>();
         b.sys_cell_base = cell_base.as<int32_t>();
+        b.sys_brick_base = brick_base.as<int32_t>();
+        b.nbricks = nbricks;
         b.nsys = nsys;
         b.nat = nat;
         b.M = M;
@@ -140,6 +144,7 @@ struct nlh_handle {
         b.cell_id_out = cell_id.p;
         b.cell_offsets_out = cell_offsets.p;
         b.pair_count = pair_count.as<int32_t>();
+        b.masks = masks.as<uint64_t>();
         b.pair_off = pair_off.as<int64_t>();
         b.scan_partials = partials.as<int64_t>();
         b.sys_pair_base = sys_pair_base.as<int64_t>();
@@ -316,6 +321,28 @@ void drop_graph(nlh_handle *h)
     h->graph = nullptr;
 }
 
+// Pair-search brick of a system: the largest of a fixed ladder of shapes
+// whose halo is expected to fit the LDS budget at this system's mean atom
+// density per cell (clipped to the cell grid).
+void choose_brick(const nlh_geometry &g, int64_t nat, int32_t out[3])
+{
+    static const int32_t ladder[][3] = {{4, 4, 4}, {4, 4, 2}, {4, 2, 2}, {2, 2, 2}, {2, 2, 1}, {2, 1, 1}, {1, 1, 1}};
+    const double per_cell = g.ncells_total > 0 ? (double)nat / (double)g.ncells_total : 0.0;
+    for (const auto &b : ladder) {
+        int32_t d[3];
+        double halo = 1.0;
+        for (int k = 0; k < 3; ++k) {
+            d[k] = (int32_t)std::min<int64_t>(b[k], g.ncells[k]);
+            halo *= (double)(d[k] + 2);
+        }
+        if (halo * per_cell * 1.25 <= (double)nlh::kBrickAtomBudget && halo <= 512.0) {
+            std::memcpy(out, d, sizeof d);
+            return;
+        }
+    }
+    out[0] = out[1] = out[2] = 1;
+}
+
 // Set up the system tables of a batch (a single build is nsys == 1) and
 // size every build-side buffer.  Uploads go through pinned staging.
 void setup_batch(nlh_handle *h, int32_t nsys, const int64_t *atom_offsets, const nlh_geometry *geoms)
@@ -325,8 +352,9 @@ void setup_batch(nlh_handle *h, int32_t nsys, const int64_t *atom_offsets, const
     h->sys_host.resize(nsys);
     h->atom_base_host.resize(nsys + 1);
     h->cell_base_host.resize(nsys + 1);
+    h->brick_base_host.resize(nsys + 1);
     h->geoms.assign(geoms, geoms + nsys);
-    int64_t cells = 0;
+    int64_t cells = 0, bricks = 0;
     if (atom_offsets[0] != 0) throw Error(NLH_EINVAL, "atom_offsets[0] must be 0");
     for (int s = 0; s < nsys; ++s) {
         validate_geometry(geoms[s]);
@@ -335,8 +363,16 @@ void setup_batch(nlh_handle *h, int32_t nsys, const int64_t *atom_offsets, const
         if (na < 0) throw Error(NLH_EINVAL, "atom_offsets must be non-decreasing");
         if (ti == 4 && na >= INT32_MAX) throw Error(NLH_EOVERFLOW, "too many atoms for Int32 indices");
         h->sys_host[s] = make_sys(geoms[s], na, atom_offsets[s], (int32_t)cells, atom_offsets[s] + s);
+        choose_brick(geoms[s], na, h->sys_host[s].brick);
+        h->sys_host[s].brick_base = (int32_t)bricks;
         h->atom_base_host[s] = atom_offsets[s];
         h->cell_base_host[s] = (int32_t)cells;
+        h->brick_base_host[s] = (int32_t)bricks;
+        {
+            const nlh::SysDev &sd = h->sys_host[s];
+            bricks += (int64_t)((sd.n[0] + sd.brick[0] - 1) / sd.brick[0]) * ((sd.n[1] + sd.brick[1] - 1) / sd.brick[1]) *
+                      ((sd.n[2] + sd.brick[2] - 1) / sd.brick[2]);
+        }
         cells += geoms[s].ncells_total;
         if (cells > INT32_MAX - 1) throw Error(NLH_EOVERFLOW, "more than 2^31-1 cells in one batch");
     }
@@ -344,6 +380,9 @@ void setup_batch(nlh_handle *h, int32_t nsys, const int64_t *atom_offsets, const
     if (nat > INT32_MAX - 1) throw Error(NLH_EOVERFLOW, "more than 2^31-1 atoms in one build");
     h->atom_base_host[nsys] = nat;
     h->cell_base_host[nsys] = (int32_t)cells;
+    h->brick_base_host[nsys] = (int32_t)bricks;
+    if (bricks > INT32_MAX - 1) throw Error(NLH_EOVERFLOW, "too many pair-search bricks in one batch");
+    h->nbricks = bricks;
     h->nsys = nsys;
     h->nat = nat;
     h->M = (int32_t)cells;
@@ -354,6 +393,7 @@ void setup_batch(nlh_handle *h, int32_t nsys, const int64_t *atom_offsets, const
     h->sys.ensure(nsys * sizeof(nlh::SysDev));
     h->atom_base.ensure((nsys + 1) * sizeof(int64_t));
     h->cell_base.ensure((nsys + 1) * sizeof(int32_t));
+    h->brick_base.ensure((nsys + 1) * sizeof(int32_t));
     h->key.ensure(n1 * 4);
     h->slot.ensure(n1 * 4);
     h->cell_count.ensure(M1 * 4, /*zero=*/true);
@@ -370,14 +410,17 @@ void setup_batch(nlh_handle *h, int32_t nsys, const int64_t *atom_offsets, const
     const size_t sys_bytes = nsys * sizeof(nlh::SysDev);
     const size_t ab_bytes = (nsys + 1) * sizeof(int64_t);
     const size_t cb_bytes = (nsys + 1) * sizeof(int32_t);
-    ensure_staging(h, sys_bytes + ab_bytes + cb_bytes + 64);
+    ensure_staging(h, sys_bytes + ab_bytes + 2 * cb_bytes + 64);
     char *st = static_cast<char *>(h->staging);
     std::memcpy(st, h->sys_host.data(), sys_bytes);
     std::memcpy(st + sys_bytes, h->atom_base_host.data(), ab_bytes);
     std::memcpy(st + sys_bytes + ab_bytes, h->cell_base_host.data(), cb_bytes);
+    std::memcpy(st + sys_bytes + ab_bytes + cb_bytes, h->brick_base_host.data(), cb_bytes);
     HIPC(hipMemcpyAsync(h->sys.p, st, sys_bytes, hipMemcpyHostToDevice, h->stream));
     HIPC(hipMemcpyAsync(h->atom_base.p, st + sys_bytes, ab_bytes, hipMemcpyHostToDevice, h->stream));
     HIPC(hipMemcpyAsync(h->cell_base.p, st + sys_bytes + ab_bytes, cb_bytes, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(h->brick_base.p, st + sys_bytes + ab_bytes + cb_bytes, cb_bytes, hipMemcpyHostToDevice,
+                        h->stream));
 }
 
 void upload_positions(nlh_handle *h, const double *X, uint32_t flags)
@@ -450,6 +493,7 @@ void ensure_pair_buffers(nlh_handle *h)
 {
     const size_t n1 = (size_t)std::max<int64_t>(h->nat, 1);
     h->pair_count.ensure(n1 * 4);
+    h->masks.ensure(n1 * 8 * nlh::kMaskChunks);
     h->pair_off.ensure((n1 + 1) * 8);
     h->first.ensure((n1 + h->nsys) * h->ti_bytes);
     h->sys_pair_base.ensure((h->nsys + 1) * 8);
@@ -609,9 +653,9 @@ int nlh_destroy(nlh_handle *h)
         drop_graph(h);
         for (hipEvent_t &e : h->ev)
             if (e) (void)hipEventDestroy(e);
-        DevBuf *bufs[] = {&h->X,        &h->sys,       &h->atom_base,  &h->cell_base,    &h->key,    &h->slot,
+        DevBuf *bufs[] = {&h->X,        &h->sys,       &h->atom_base,  &h->cell_base, &h->brick_base,    &h->key,    &h->slot,
                           &h->cell_count, &h->cell_start, &h->tmp,      &h->rec,          &h->wbig,   &h->perm,
-                          &h->cell_id,  &h->cell_offsets, &h->pair_count, &h->pair_off,   &h->partials,
+                          &h->cell_id,  &h->cell_offsets, &h->pair_count, &h->masks, &h->pair_off,   &h->partials,
                           &h->sys_pair_base, &h->first, &h->i,          &h->j,            &h->S,      &h->R,
                           &h->status,   &h->Xsorted,   &h->nb_j,       &h->nb_R,         &h->nb_S,   &h->nb_count};
         for (DevBuf *b : bufs) b->release();

@@ -19,6 +19,8 @@ constexpr int kWavesPerPairsBlock = kPairsBlock / kWave;
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;            // elements per thread in a scan tile
 constexpr int kScanTile = kScanThreads * kScanItems;
+constexpr int kMaskChunks = 8;            // hit masks kept per atom (512 candidates)
+constexpr int kBrickAtomBudget = 1536;    // expected halo atoms per brick (LDS holds 2048)
 
 // One system of a batch (a single build is a batch of one).
 struct SysDev {
@@ -33,6 +35,8 @@ struct SysDev {
     int32_t cell_base;  // first global cell index
     int32_t M;          // cells in this system
     int64_t first_base; // offset of this system's `first` array (atom_base + sys index)
+    int32_t brick[3];   // home cells per brick along each axis (pair search work unit)
+    int32_t brick_base; // first global brick index
 };
 
 // Sorted atom record: position, original (system-local, 0-based) index and
@@ -144,7 +148,9 @@ struct PipelineBufs {
     const SysDev *sys;          // nsys
     const int64_t *sys_atom_base;   // nsys + 1
     const int32_t *sys_cell_base;   // nsys + 1
+    const int32_t *sys_brick_base;  // nsys + 1
     int32_t nsys;
+    int64_t nbricks;            // total pair-search bricks
     int64_t nat;                // total atoms
     int32_t M;                  // total cells
     // build
@@ -160,6 +166,7 @@ struct PipelineBufs {
     void *cell_offsets_out;     // M + nsys, TI, 1-based per system
     // pairs
     int32_t *pair_count;        // nat
+    uint64_t *masks;            // kMaskChunks * nat hit masks
     int64_t *pair_off;          // nat + 1
     int64_t *scan_partials;     // scan block sums
     int64_t *sys_pair_base;     // nsys + 1 (device)
@@ -175,9 +182,6 @@ void launch_build(const PipelineBufs &b, hipStream_t st);
 void launch_count(const PipelineBufs &b, hipStream_t st);
 void launch_scan_pairs(const PipelineBufs &b, hipStream_t st);
 void launch_fill(const PipelineBufs &b, bool want_R, hipStream_t st);
-void launch_iexpand(const PipelineBufs &b, hipStream_t st);
 void launch_sorted_positions(const PipelineBufs &b, double *Xs, hipStream_t st);
-void launch_neighbours_one(const PipelineBufs &b, int sys, int64_t atom, int64_t cap, void *j,
-                           double *R, void *S, int64_t *count, hipStream_t st);
 
 }  // namespace nlh

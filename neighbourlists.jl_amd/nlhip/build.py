@@ -15,7 +15,7 @@ REPO = os.path.dirname(ROOT)
 INCLUDE = os.path.join(REPO, "include")
 OUT = os.path.join(PKG_DIR, "libnlhip.so")
 
-SOURCES = ["api.hip", "kernels.hip"]
+SOURCES = ["api.hip", "kernels.hip", "pairs.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

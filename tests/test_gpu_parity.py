@@ -162,3 +162,18 @@ def test_batch_matches_single():
         ref = oracle.neighbour_list(sysm.X, sysm.cutoff, sysm.cell, sysm.pbc, want_R=True)
         pl = res.pairlist(s)
         _same(pl, ref)
+
+
+@pytest.mark.parametrize("pbc", ALL_PBC)
+def test_bricks_medium(pbc):
+    # 20k atoms: 14^3 cells, many full 4x4x4 bricks plus clipped edge bricks
+    s = workloads.random_cubic(20000, density=0.05, seed=61)
+    print(f"bricks pbc={pbc}", flush=True)
+    _check(s.X, 5.0, s.cell, pbc)
+
+
+def test_bricks_dense_triclinic():
+    C = np.array([[40.0, 6.0, 3.0], [0.0, 38.0, 5.0], [0.0, 0.0, 36.0]])
+    s = workloads.random_in_cell(4000, C, seed=62)
+    _check(s.X, 4.0, C, (True, True, True))
+    _check(s.X, 4.0, C, (True, False, True))
