@@ -90,7 +90,7 @@ struct nlh_handle {
     int device = 0;
     hipStream_t stream = nullptr;
     DevBuf X, sys, atom_base, cell_base, brick_base, key, slot, cell_count, cell_start, tmp, rec, wbig, perm, cell_id,
-        cell_offsets, pair_count, masks, pair_off, partials, sys_pair_base, first, i, j, S, R, status, Xsorted, nb_j, nb_R,
+        cell_offsets, pair_count, inv_perm, slots, slotsR, pair_off, partials, sys_pair_base, first, i, j, S, R, status, Xsorted, nb_j, nb_R,
         nb_S, nb_count;
     nlh::Status *status_host = nullptr;   // pinned
     void *staging = nullptr;              // pinned upload staging for system tables
@@ -103,6 +103,8 @@ struct nlh_handle {
     bool built = false, materialized = false, have_R = false;
     int64_t npairs = 0;
     int64_t pair_capacity = 0;
+    int32_t slot_cap = 0;       // neighbour slots per atom
+    bool wide = false;          // 16-byte slot entries (cell shifts beyond +-511)
     const double *X_dev = nullptr;
     std::vector<nlh::SysDev> sys_host;
     std::vector<int64_t> atom_base_host;
@@ -144,7 +146,10 @@ struct nlh_handle {
         b.cell_id_out = cell_id.p;
         b.cell_offsets_out = cell_offsets.p;
         b.pair_count = pair_count.as<int32_t>();
-        b.masks = masks.as<uint64_t>();
+        b.inv_perm = inv_perm.as<int32_t>();
+        b.slots = slots.p;
+        b.slotsR = slotsR.as<double>();
+        b.slot_cap = slot_cap;
         b.pair_off = pair_off.as<int64_t>();
         b.scan_partials = partials.as<int64_t>();
         b.sys_pair_base = sys_pair_base.as<int64_t>();
@@ -399,6 +404,7 @@ void setup_batch(nlh_handle *h, int32_t nsys, const int64_t *atom_offsets, const
     h->cell_count.ensure(M1 * 4, /*zero=*/true);
     h->cell_start.ensure(M1 * 4);
     h->tmp.ensure(n1 * 4);
+    h->inv_perm.ensure(n1 * 4);
     h->rec.ensure(n1 * sizeof(nlh::Rec));
     h->wbig.ensure(n1 * 12);
     h->perm.ensure(n1 * ti);
@@ -475,6 +481,30 @@ int64_t estimate_pairs(const nlh_handle *h)
     return (int64_t)std::min<long double>(est, 4.0e11L) + 1024;
 }
 
+// Expected neighbours per atom, largest over the batch (a slot-size guess;
+// an underestimate costs one repeat of the count and fill passes).
+int32_t estimate_slot_cap(const nlh_handle *h)
+{
+    double mx = 0;
+    for (int s = 0; s < h->nsys; ++s) {
+        const nlh_geometry &g = h->geoms[s];
+        const double vol = std::fabs(det3(g.cell));
+        const double nat = (double)h->sys_host[s].nat;
+        if (nat <= 0 || !(vol > 0)) continue;
+        mx = std::max(mx, nat / vol * 4.18879020478639 * g.cutoff * g.cutoff * g.cutoff);
+    }
+    const double c = std::min(mx * 1.5 + 16.0, 1.0e6);
+    return (int32_t)((c + 7) / 8) * 8;
+}
+
+void ensure_slots(nlh_handle *h, int32_t cap, bool want_R)
+{
+    h->slot_cap = std::max(h->slot_cap, cap);
+    const size_t n1 = (size_t)std::max<int64_t>(h->nat, 1) * (size_t)h->slot_cap;
+    h->slots.ensure(n1 * (h->wide ? 16 : 8));
+    if (want_R) h->slotsR.ensure(n1 * 24);
+}
+
 void ensure_pair_outputs(nlh_handle *h, int64_t cap, bool want_R)
 {
     const int ti = h->ti_bytes;
@@ -493,7 +523,6 @@ void ensure_pair_buffers(nlh_handle *h)
 {
     const size_t n1 = (size_t)std::max<int64_t>(h->nat, 1);
     h->pair_count.ensure(n1 * 4);
-    h->masks.ensure(n1 * 8 * nlh::kMaskChunks);
     h->pair_off.ensure((n1 + 1) * 8);
     h->first.ensure((n1 + h->nsys) * h->ti_bytes);
     h->sys_pair_base.ensure((h->nsys + 1) * 8);
@@ -537,26 +566,34 @@ void materialize(nlh_handle *h, uint32_t flags)
         return;
     }
     ensure_pair_outputs(h, std::max(h->pair_capacity, estimate_pairs(h)), want_R);
+    ensure_slots(h, std::max(h->slot_cap, estimate_slot_cap(h)), want_R);
     nlh::PipelineBufs b = h->bufs();
     mark(h, 1);
-    nlh::launch_count(b, h->stream);
+    nlh::launch_count(b, want_R, h->wide, h->stream);
     mark(h, 2);
     nlh::launch_scan_pairs(b, h->stream);
     mark(h, 3);
-    nlh::launch_fill(b, want_R, h->stream);
+    nlh::launch_fill(b, want_R, h->wide, h->stream);
     mark(h, 4);
     h->ev_pairs = h->profile;
     HIPC(hipGetLastError());
     finish_status(h);
     check_positions(h);
-    if (h->status_host->overflow) {   // grow and redo the scan + fill
-        ensure_pair_outputs(h, h->status_host->npairs, want_R);
+    const nlh::Status &st = *h->status_host;
+    if (st.npairs > h->pair_capacity || st.max_count > h->slot_cap || (st.wide && !h->wide)) {
+        // grow the outputs / slots (or widen the slot entries) and repeat count + fill
+        // (the counts, offsets and totals of the first pass stay valid)
+        if (st.wide) h->wide = true;
+        ensure_pair_outputs(h, std::max<int64_t>(st.npairs, 1), want_R);
+        h->slots.release();
+        ensure_slots(h, (int32_t)((std::max(st.max_count, 1) + 7) / 8) * 8, want_R);
         b = h->bufs();
-        nlh::launch_scan_pairs(b, h->stream);
-        nlh::launch_fill(b, want_R, h->stream);
+        nlh::launch_count(b, want_R, h->wide, h->stream);
+        nlh::launch_fill(b, want_R, h->wide, h->stream);
         HIPC(hipGetLastError());
         finish_status(h);
-        if (h->status_host->overflow) throw Error(NLH_EDEVICE, "pair capacity still exceeded after growth");
+        if (h->status_host->npairs > h->pair_capacity || h->status_host->max_count > h->slot_cap)
+            throw Error(NLH_EDEVICE, "pair capacity still exceeded after growth");
     }
     h->npairs = h->status_host->npairs;
     h->have_R = want_R;
@@ -655,7 +692,7 @@ int nlh_destroy(nlh_handle *h)
             if (e) (void)hipEventDestroy(e);
         DevBuf *bufs[] = {&h->X,        &h->sys,       &h->atom_base,  &h->cell_base, &h->brick_base,    &h->key,    &h->slot,
                           &h->cell_count, &h->cell_start, &h->tmp,      &h->rec,          &h->wbig,   &h->perm,
-                          &h->cell_id,  &h->cell_offsets, &h->pair_count, &h->masks, &h->pair_off,   &h->partials,
+                          &h->cell_id,  &h->cell_offsets, &h->pair_count, &h->inv_perm, &h->slots, &h->slotsR, &h->pair_off,   &h->partials,
                           &h->sys_pair_base, &h->first, &h->i,          &h->j,            &h->S,      &h->R,
                           &h->status,   &h->Xsorted,   &h->nb_j,       &h->nb_R,         &h->nb_S,   &h->nb_count};
         for (DevBuf *b : bufs) b->release();

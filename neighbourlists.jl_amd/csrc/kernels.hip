@@ -147,6 +147,15 @@ __global__ __launch_bounds__(kScanThreads) void scan_pairs_down_kernel(
         status->npairs = tot;
         status->overflow = tot > capacity ? 1 : 0;
     }
+    int32_t mx = 0;   // largest per-atom count (slot capacity check)
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q) mx = v[q] > mx ? v[q] : mx;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const int32_t o = __shfl_xor(mx, off, kWave);
+        mx = o > mx ? o : mx;
+    }
+    if (lane_id() == 0 && mx > 0) atomicMax(&status->max_count, mx);
     int64_t run = boff + ex;
     const int64_t b = base + (int64_t)threadIdx.x * kScanItems;
     for (int q = 0; q < kScanItems; ++q) {
@@ -206,8 +215,8 @@ __global__ __launch_bounds__(256) void place_kernel(const double *__restrict__ X
                                                     const int32_t *__restrict__ sys_cell_base, int nsys,
                                                     const int32_t *__restrict__ key, const int32_t *__restrict__ cell_start,
                                                     const int32_t *__restrict__ tmp, int64_t nat, Rec *__restrict__ rec,
-                                                    int32_t *__restrict__ wbig, void *__restrict__ perm_out,
-                                                    void *__restrict__ cell_id_out)
+                                                    int32_t *__restrict__ wbig, int32_t *__restrict__ inv_perm,
+                                                    void *__restrict__ perm_out, void *__restrict__ cell_id_out)
 {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nat) return;
@@ -227,6 +236,7 @@ __global__ __launch_bounds__(256) void place_kernel(const double *__restrict__ X
     r.x = x; r.y = y; r.z = z; r.j = j;
     r.w = pack_wind(w[0], w[1], w[2]);
     rec[p] = r;
+    inv_perm[e] = p;
     if (r.w & kWindBig) {
         wbig[3 * (int64_t)p] = w[0];
         wbig[3 * (int64_t)p + 1] = w[1];
@@ -286,7 +296,7 @@ void launch_build(const PipelineBufs &b, hipStream_t st)
         NLH_TI_DISPATCH(b.ti_bytes,
                         hipLaunchKernelGGL(place_kernel<TI>, dim3(blocks_for(b.nat, 256)), dim3(256), 0, st, b.X,
                                            b.sys, b.sys_cell_base, b.nsys, b.key, b.cell_start, b.tmp, b.nat, b.rec,
-                                           b.wbig, b.perm_out, b.cell_id_out));
+                                           b.wbig, b.inv_perm, b.perm_out, b.cell_id_out));
     }
 }
 

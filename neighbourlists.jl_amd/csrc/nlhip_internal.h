@@ -19,7 +19,6 @@ constexpr int kWavesPerPairsBlock = kPairsBlock / kWave;
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;            // elements per thread in a scan tile
 constexpr int kScanTile = kScanThreads * kScanItems;
-constexpr int kMaskChunks = 8;            // hit masks kept per atom (512 candidates)
 constexpr int kBrickAtomBudget = 1536;    // expected halo atoms per brick (LDS holds 2048)
 
 // One system of a batch (a single build is a batch of one).
@@ -65,10 +64,11 @@ __host__ __device__ inline void unpack_wind(uint32_t w, int32_t &wx, int32_t &wy
 // Device-side status block, copied to pinned host memory once per build.
 struct Status {
     int64_t npairs;          // total pairs over the batch
-    int64_t max_sys_pairs;   // largest per-system pair count (TI overflow check)
+    int64_t reserved;
     int32_t overflow;        // pair capacity exceeded: outputs not written
     int32_t bad_position;    // non-finite / out-of-range position seen
-    int32_t pad[2];
+    int32_t wide;            // a cell shift does not fit a packed slot entry
+    int32_t max_count;       // largest per-atom neighbour count
 };
 
 // ---------------------------------------------------------------------------
@@ -166,7 +166,10 @@ struct PipelineBufs {
     void *cell_offsets_out;     // M + nsys, TI, 1-based per system
     // pairs
     int32_t *pair_count;        // nat
-    uint64_t *masks;            // kMaskChunks * nat hit masks
+    int32_t *inv_perm;          // nat: original -> sorted index
+    void *slots;                // nat * slot_cap slot entries
+    double *slotsR;             // 3 * nat * slot_cap (R requested)
+    int32_t slot_cap;
     int64_t *pair_off;          // nat + 1
     int64_t *scan_partials;     // scan block sums
     int64_t *sys_pair_base;     // nsys + 1 (device)
@@ -179,9 +182,9 @@ struct PipelineBufs {
 };
 
 void launch_build(const PipelineBufs &b, hipStream_t st);
-void launch_count(const PipelineBufs &b, hipStream_t st);
+void launch_count(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st);
 void launch_scan_pairs(const PipelineBufs &b, hipStream_t st);
-void launch_fill(const PipelineBufs &b, bool want_R, hipStream_t st);
+void launch_fill(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st);
 void launch_sorted_positions(const PipelineBufs &b, double *Xs, hipStream_t st);
 
 }  // namespace nlh

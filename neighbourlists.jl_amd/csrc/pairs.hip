@@ -1,24 +1,26 @@
-// Pair search of the sort-based neighbour list: count (with hit masks) and
-// fill, the MI355X counterparts of count_neighbours_kernel! and
-// fill_pairs_kernel! (src/gpu_kernels.jl:132-180) around the inner loop
-// _for_each_neighbor_pair (src/gpu_kernels.jl:58-101).
+// Pair search of the sort-based neighbour list: count and fill, the MI355X
+// counterparts of count_neighbours_kernel! and fill_pairs_kernel!
+// (src/gpu_kernels.jl:132-180) around the inner loop _for_each_neighbor_pair
+// (src/gpu_kernels.jl:58-101).
 //
-// Work unit: a brick of up to 4x4x4 home cells per 256-thread workgroup.
-// The brick and its one-cell halo are staged in LDS in one load phase (every
-// global load of the phase in flight together), then each wave takes home
-// cells and tests candidates from LDS only:
-//   - candidates of a home cell are its 27 neighbour cells flattened in the
-//     reference order (dz, dy, dx; gpu_kernels.jl:69-71) -- nine contiguous
-//     LDS rows -- and cut into 64-candidate chunks, one candidate per lane;
+// count_kernel -- work unit: a brick of up to 4x4x4 home cells per
+// 256-thread workgroup.  The brick and its one-cell halo are staged in LDS in
+// one load phase (every global load of the phase in flight together), then
+// each wave takes home cells and tests candidates from LDS only:
+//   - the candidates of a home cell are its 27 neighbour cells flattened in
+//     the reference order (dz, dy, dx; gpu_kernels.jl:69-71) -- nine
+//     contiguous LDS rows -- cut into 64-candidate chunks, one per lane;
 //   - the home atoms stream through the wave as uniform values; each
-//     (home atom, chunk) test is one ballot, so a home atom's neighbours keep
-//     exactly the reference traversal order;
-//   - the count pass stores each ballot (kMaskChunks chunks per atom); the
-//     fill pass expands stored ballots with one lane per hit, so consecutive
-//     lanes write an atom's run of (i, j, S) contiguously.
+//     (home atom, chunk) test is one ballot, and the hit lanes append their
+//     (j, S) to the home atom's slot at its running count, so every slot
+//     holds the atom's neighbours in exactly the reference traversal order.
 // Bricks whose neighbour reach exceeds one cell, whose halo does not fit
 // LDS, or whose windings do not pack take a per-wave path that reads global
-// memory (one home cell per wave), with the same enumeration and output.
+// memory (one home cell per wave) with the same enumeration and output.
+//
+// fill_kernel -- atoms in their original order, one lane per pair: each wave
+// copies the slots of 64 consecutive atoms into one contiguous range of the
+// PairList arrays (i, j, S and optionally R), so its stores are linear.
 #include "device_util.h"
 
 namespace nlh {
@@ -33,34 +35,6 @@ __device__ __forceinline__ uint64_t shfl64(uint64_t v, int src)
 {
     const uint32_t lo = __shfl((uint32_t)v, src, kWave), hi = __shfl((uint32_t)(v >> 32), src, kWave);
     return ((uint64_t)hi << 32) | lo;
-}
-
-__device__ __forceinline__ double shfl_d(double v, int src)
-{
-    return __longlong_as_double((long long)shfl64((uint64_t)__double_as_longlong(v), src));
-}
-
-// Position of the r-th (0-based) set bit of m.
-__device__ __forceinline__ int nth_set_bit(uint64_t m, int r)
-{
-    int pos = 0;
-    uint32_t w = (uint32_t)m;
-    int c = __popc(w);
-    if (r >= c) {
-        r -= c;
-        w = (uint32_t)(m >> 32);
-        pos = 32;
-    }
-    c = __popc(w & 0xffffu);
-    if (r >= c) { r -= c; w >>= 16; pos += 16; }
-    c = __popc(w & 0xffu);
-    if (r >= c) { r -= c; w >>= 8; pos += 8; }
-    c = __popc(w & 0xfu);
-    if (r >= c) { r -= c; w >>= 4; pos += 4; }
-    c = __popc(w & 0x3u);
-    if (r >= c) { r -= c; w >>= 2; pos += 2; }
-    if (r >= (int)(w & 1u)) pos += 1;
-    return pos;
 }
 
 // (C' * S)_k = sum_m C[m,k] S_m   (gpu_kernels.jl:89)
@@ -86,17 +60,83 @@ __device__ __forceinline__ void rec_winding(const Rec &r, const int32_t *__restr
 }
 
 constexpr int kChunksPerPass = 4;   // candidate chunks a lane holds at once
+constexpr double kFar = 1.0e200;    // invalid candidate: r^2 overflows to +inf, never a hit
 
 struct PairsArgs {
     const SysDev *__restrict__ sys;
     const int32_t *__restrict__ sys_cell_base;
     int nsys;
     int32_t M;
-    int64_t nat;                // mask stride
+    int64_t nat;
     const int32_t *__restrict__ cell_start;
     const Rec *__restrict__ rec;
     const int32_t *__restrict__ wbig;
 };
+
+// ---------------------------------------------------------------------------
+// slots: each atom's neighbours (j, S) in traversal order, `cap` per atom,
+// indexed by sorted atom position.  Narrow entries pack S into 3 x 10 bits
+// (8 bytes); wide entries keep four int32.
+// ---------------------------------------------------------------------------
+
+template <bool WIDE>
+struct SlotEntry;
+template <>
+struct SlotEntry<false> {
+    using T = uint64_t;
+    __device__ static void get(T e, int32_t &j, int32_t &Sx, int32_t &Sy, int32_t &Sz)
+    {
+        j = (int32_t)(uint32_t)e;
+        unpack_wind((uint32_t)(e >> 32), Sx, Sy, Sz);
+    }
+};
+template <>
+struct SlotEntry<true> {
+    using T = int4;
+    __device__ static void get(T e, int32_t &j, int32_t &Sx, int32_t &Sy, int32_t &Sz)
+    {
+        j = e.x; Sx = e.y; Sy = e.z; Sz = e.w;
+    }
+};
+
+struct SlotArgs {
+    void *__restrict__ slots;
+    double *__restrict__ slotsR;     // 3 doubles per entry (NLH_WANT_R)
+    int32_t cap;
+    Status *__restrict__ status;
+};
+
+// Append the hits of one (home atom, chunk) ballot to the home atom's slot.
+// sbase = sorted index of the home atom * cap; ca = its running count
+// (wave-uniform).  t_pk: this lane's packed s - w_j (valid when tfits);
+// (tx, ty, tz) the same unpacked; w_i the home atom's winding, wzero when 0.
+template <bool WANT_R, bool WIDE>
+__device__ __forceinline__ void emit_hits(const SlotArgs &O, uint64_t mask, int64_t sbase, int32_t &ca, int32_t cj,
+                                          uint32_t t_pk, int32_t tx, int32_t ty, int32_t tz, int32_t wix, int32_t wiy,
+                                          int32_t wiz, bool wzero, double Rx, double Ry, double Rz)
+{
+    if (mask == 0) return;   // wave-uniform
+    const int lane = lane_id();
+    const int32_t rank = ca + (int32_t)mbcnt64(mask);
+    if (((mask >> lane) & 1) && rank < O.cap) {
+        const int64_t sidx = sbase + rank;
+        if (WIDE) {
+            reinterpret_cast<int4 *>(O.slots)[sidx] = make_int4(cj, tx + wix, ty + wiy, tz + wiz);
+        } else {
+            // S = s_loop + w_i - w_j = t + w_i  (gpu_kernels.jl:88)
+            const uint32_t spk = wzero ? t_pk : pack_wind(tx + wix, ty + wiy, tz + wiz);
+            if (spk & kWindBig) O.status->wide = 1;
+            reinterpret_cast<uint64_t *>(O.slots)[sidx] = (uint64_t)(uint32_t)cj | ((uint64_t)spk << 32);
+        }
+        if (WANT_R) {
+            double *R = O.slotsR + 3 * sidx;
+            R[0] = Rx;
+            R[1] = Ry;
+            R[2] = Rz;
+        }
+    }
+    ca += __popcll(mask);
+}
 
 // ---------------------------------------------------------------------------
 // per-wave path: one home cell per wave, candidates read from global memory
@@ -204,55 +244,12 @@ __device__ __forceinline__ Cand load_candidate(const PairsArgs &A, int k, int to
     return c;
 }
 
-// Distance tests of home atoms hg .. hg+nh-1 (sorted indices) against the
-// held chunks g in [g0, nch): mine[g] gets home atom a's hit ballot in lane a
-// (gpu_kernels.jl:79-93, same evaluation order).
-__device__ __forceinline__ void test_chunks(const PairsArgs &A, const Cand (&c)[kChunksPerPass], int g0, int nch,
-                                            int32_t hg, int nh, const double *__restrict__ Cm, double rc2,
-                                            uint64_t (&mine)[kChunksPerPass])
-{
-    uint64_t valid[kChunksPerPass], zs[kChunksPerPass];
-    double cx[kChunksPerPass], cy[kChunksPerPass], cz[kChunksPerPass];
-#pragma unroll
-    for (int g = 0; g < kChunksPerPass; ++g) {
-        cx[g] = cy[g] = cz[g] = 0.0;
-        valid[g] = zs[g] = 0;
-        if (g >= g0 && g < nch) {
-            valid[g] = __ballot(c[g].valid);
-            zs[g] = __ballot(c[g].zero_shift);
-        }
-    }
-    uint32_t wcache = 0xffffffffu;
-    Rec hn = A.rec[hg];
-    for (int a = 0; a < nh; ++a) {
-        const Rec h = hn;
-        if (a + 1 < nh) hn = A.rec[hg + a + 1];
-        if (h.w != wcache || (h.w & kWindBig)) {   // w_i changed: refresh C' * S per lane
-            int32_t wix, wiy, wiz;
-            rec_winding(h, A.wbig, hg + a, wix, wiy, wiz);
-#pragma unroll
-            for (int g = 0; g < kChunksPerPass; ++g)
-                if (g >= g0 && g < nch) cell_shift(Cm, c[g].tx + wix, c[g].ty + wiy, c[g].tz + wiz, cx[g], cy[g], cz[g]);
-            wcache = h.w;
-        }
-#pragma unroll
-        for (int g = 0; g < kChunksPerPass; ++g) {
-            if (g >= g0 && g < nch) {
-                const double Rx = (c[g].x - h.x) + cx[g];
-                const double Ry = (c[g].y - h.y) + cy[g];
-                const double Rz = (c[g].z - h.z) + cz[g];
-                const double r2 = (Rx * Rx + Ry * Ry) + Rz * Rz;
-                const uint64_t in = __ballot(r2 < rc2);
-                const uint64_t self = __ballot(c[g].j == h.j) & zs[g];   // _is_self_interaction
-                const uint64_t mask = in & valid[g] & ~self;
-                mine[g] = lane_id() == a ? mask : mine[g];
-            }
-        }
-    }
-}
-
+// Per-wave count of one home cell from global memory: test every (home
+// atom, chunk) and append hits to the slots (the brick path's fallback, and
+// the path for neighbour reach beyond one cell).
+template <bool WANT_R, bool WIDE>
 __device__ __noinline__ void count_cell(const PairsArgs &A, const SysDev &sd, int cell, int32_t *__restrict__ pair_count,
-                           uint64_t *__restrict__ masks)
+                                        const SlotArgs &O)
 {
     HomeCell H;
     home_cell(A, sd, cell, H);
@@ -263,8 +260,7 @@ __device__ __noinline__ void count_cell(const PairsArgs &A, const SysDev &sd, in
     for (int32_t hg = H.hs; hg < H.he; hg += kWave) {
         const int nh = sgpr(min(kWave, H.he - hg));
         const int32_t hj_lane = lane < nh ? A.rec[hg + lane].j : 0;
-        int cnt = 0;
-        int q = 0;
+        int32_t cnt_lane = 0;   // lane a: running count of home atom hg + a
         for (int sb = 0; sb < H.nseg; sb += kWave) {
             int excl, start_g;
             uint32_t shp;
@@ -272,125 +268,57 @@ __device__ __noinline__ void count_cell(const PairsArgs &A, const SysDev &sd, in
             for (int k0 = 0; k0 < total; k0 += kChunksPerPass * kWave) {
                 const int nch = sgpr(min(kChunksPerPass, (total - k0 + kWave - 1) / kWave));
                 Cand c[kChunksPerPass];
-                uint64_t mine[kChunksPerPass];
-#pragma unroll
-                for (int g = 0; g < kChunksPerPass; ++g) {
-                    mine[g] = 0;
-                    if (g < nch) c[g] = load_candidate(A, k0 + g * kWave + lane, total, excl, start_g, shp);
-                }
-                test_chunks(A, c, 0, nch, hg, nh, Cm, rc2, mine);
+                uint32_t tpk[kChunksPerPass];
+                int32_t key[kChunksPerPass];
 #pragma unroll
                 for (int g = 0; g < kChunksPerPass; ++g) {
                     if (g < nch) {
-                        cnt += __popcll(mine[g]);
-                        if (q + g < kMaskChunks && lane < nh) masks[(int64_t)(q + g) * A.nat + hg + lane] = mine[g];
+                        c[g] = load_candidate(A, k0 + g * kWave + lane, total, excl, start_g, shp);
+                        if (!c[g].valid) c[g].x = c[g].y = c[g].z = kFar;
+                        tpk[g] = pack_wind(c[g].tx, c[g].ty, c[g].tz);
+                        key[g] = (c[g].valid && c[g].zero_shift) ? c[g].j : -3;
+                    } else {
+                        c[g].x = c[g].y = c[g].z = kFar;
+                        c[g].j = -1;
+                        c[g].tx = c[g].ty = c[g].tz = 0;
+                        tpk[g] = 0;
+                        key[g] = -3;
                     }
                 }
-                q += nch;
-            }
-        }
-        if (lane < nh) pair_count[H.abase + hj_lane] = cnt;
-    }
-}
-
-// Expand one chunk's hit masks with one lane per hit.  mine: lane a's mask
-// (home atom a); cursor: lane a's write position; candidate data per lane.
-template <bool WANT_R, typename TI>
-__device__ __forceinline__ void expand_chunk(uint64_t mine, int64_t &cursor, const Cand &c, int32_t hj_l, int32_t wx_l,
-                                             int32_t wy_l, int32_t wz_l, double hx_l, double hy_l, double hz_l,
-                                             const double *__restrict__ Cm, TI *__restrict__ i_out,
-                                             TI *__restrict__ j_out, TI *__restrict__ S_out, double *__restrict__ R_out)
-{
-    const int lane = lane_id();
-    const int pc = __popcll(mine);
-    const int incl = wave_incl_scan(pc);
-    const int tot = sgpr(__shfl(incl, kWave - 1, kWave));
-    const int exh = incl - pc;
-    for (int h0 = 0; h0 < tot; h0 += kWave) {
-        const int L = h0 + lane;
-        int a = 0;   // home atom: largest a with exh[a] <= L
+                double cx[kChunksPerPass], cy[kChunksPerPass], cz[kChunksPerPass];
+                uint32_t wcache = 0xffffffffu;
+                int32_t wix = 0, wiy = 0, wiz = 0;
+                for (int a = 0; a < nh; ++a) {
+                    const Rec h = A.rec[hg + a];   // wave-uniform: scalar loads
+                    if (h.w != wcache || (h.w & kWindBig)) {
+                        rec_winding(h, A.wbig, hg + a, wix, wiy, wiz);
 #pragma unroll
-        for (int step = 32; step >= 1; step >>= 1) {
-            const int e2 = __shfl(exh, a + step, kWave);
-            if (e2 <= L) a += step;
-        }
-        const int r = L - __shfl(exh, a, kWave);
-        const int l = nth_set_bit(shfl64(mine, a), r);
-        const int32_t cj = __shfl(c.j, l, kWave);
-        const int32_t Sx = __shfl(c.tx, l, kWave) + __shfl(wx_l, a, kWave);
-        const int32_t Sy = __shfl(c.ty, l, kWave) + __shfl(wy_l, a, kWave);
-        const int32_t Sz = __shfl(c.tz, l, kWave) + __shfl(wz_l, a, kWave);
-        const int32_t hj = __shfl(hj_l, a, kWave);
-        const int64_t pos = (int64_t)shfl64((uint64_t)cursor, a) + r;
-        double Rx = 0, Ry = 0, Rz = 0;
-        if (WANT_R) {
-            double cx, cy, cz;
-            cell_shift(Cm, Sx, Sy, Sz, cx, cy, cz);
-            Rx = (shfl_d(c.x, l) - shfl_d(hx_l, a)) + cx;
-            Ry = (shfl_d(c.y, l) - shfl_d(hy_l, a)) + cy;
-            Rz = (shfl_d(c.z, l) - shfl_d(hz_l, a)) + cz;
-        }
-        if (L < tot) {
-            i_out[pos] = (TI)(hj + 1);
-            j_out[pos] = (TI)(cj + 1);
-            TI *S = S_out + 3 * pos;
-            S[0] = (TI)Sx;
-            S[1] = (TI)Sy;
-            S[2] = (TI)Sz;
-            if (WANT_R) {
-                double *R = R_out + 3 * pos;
-                R[0] = Rx;
-                R[1] = Ry;
-                R[2] = Rz;
-            }
-        }
-    }
-    cursor += pc;
-}
-
-template <bool WANT_R, typename TI>
-__device__ __noinline__ void fill_cell(const PairsArgs &A, const SysDev &sd, int cell, const uint64_t *__restrict__ masks,
-                          const int64_t *__restrict__ pair_off, TI *__restrict__ i_out, TI *__restrict__ j_out,
-                          TI *__restrict__ S_out, double *__restrict__ R_out)
-{
-    HomeCell H;
-    home_cell(A, sd, cell, H);
-    if (H.hs >= H.he) return;
-    const int lane = lane_id();
-    const double *__restrict__ Cm = sd.C;
-    const double rc2 = sd.rc2;
-    for (int32_t hg = H.hs; hg < H.he; hg += kWave) {
-        const int nh = sgpr(min(kWave, H.he - hg));
-        Rec hl;
-        if (lane < nh) hl = A.rec[hg + lane];
-        else { hl.x = hl.y = hl.z = 0.0; hl.j = 0; hl.w = 0; }
-        int32_t wx = 0, wy = 0, wz = 0;
-        if (lane < nh) rec_winding(hl, A.wbig, hg + lane, wx, wy, wz);
-        int64_t cursor = lane < nh ? pair_off[H.abase + hl.j] : 0;
-        int q = 0;
-        for (int sb = 0; sb < H.nseg; sb += kWave) {
-            int excl, start_g;
-            uint32_t shp;
-            const int total = segment_batch(A, H, sb, excl, start_g, shp);
-            for (int k0 = 0; k0 < total; k0 += kWave, ++q) {
-                Cand c[kChunksPerPass];
-                c[0] = load_candidate(A, k0 + lane, total, excl, start_g, shp);
-                uint64_t mine[kChunksPerPass] = {0, 0, 0, 0};
-                if (q < kMaskChunks) {
-                    if (lane < nh) mine[0] = masks[(int64_t)q * A.nat + hg + lane];
-                } else {
-                    test_chunks(A, c, 0, 1, hg, nh, Cm, rc2, mine);
+                        for (int g = 0; g < kChunksPerPass; ++g)
+                            cell_shift(Cm, c[g].tx + wix, c[g].ty + wiy, c[g].tz + wiz, cx[g], cy[g], cz[g]);
+                        wcache = h.w;
+                    }
+                    const bool wzero = (wix | wiy | wiz) == 0;
+                    int32_t ca = __builtin_amdgcn_readlane(cnt_lane, a);
+                    const int64_t sbase = (int64_t)(hg + a) * O.cap;
+#pragma unroll
+                    for (int g = 0; g < kChunksPerPass; ++g) {
+                        if (g < nch) {
+                            const double Rx = (c[g].x - h.x) + cx[g];
+                            const double Ry = (c[g].y - h.y) + cy[g];
+                            const double Rz = (c[g].z - h.z) + cz[g];
+                            const double r2 = (Rx * Rx + Ry * Ry) + Rz * Rz;
+                            const uint64_t mask = __ballot((r2 < rc2) & (key[g] != h.j));
+                            emit_hits<WANT_R, WIDE>(O, mask, sbase, ca, c[g].j, tpk[g], c[g].tx, c[g].ty, c[g].tz, wix,
+                                                    wiy, wiz, wzero, Rx, Ry, Rz);
+                        }
+                    }
+                    cnt_lane = lane == a ? ca : cnt_lane;
                 }
-                expand_chunk<WANT_R, TI>(mine[0], cursor, c[0], hl.j, wx, wy, wz, hl.x, hl.y, hl.z, Cm, i_out, j_out,
-                                         S_out, R_out);
             }
         }
+        if (lane < nh) pair_count[H.abase + hj_lane] = cnt_lane;
     }
 }
-
-// ---------------------------------------------------------------------------
-// brick path
-// ---------------------------------------------------------------------------
 
 constexpr int kBrickThreads = 256;
 constexpr int kBrickWaves = kBrickThreads / kWave;
@@ -580,50 +508,54 @@ __device__ __forceinline__ int row_index(const HomeRows &R, int k)
     return __shfl(R.rstart, m, kWave) + (k - __shfl(R.rex, m, kWave));
 }
 
+
+template <bool WANT_R, bool WIDE>
 __global__ __launch_bounds__(kBrickThreads) void count_kernel(
     const SysDev *__restrict__ sys, const int32_t *__restrict__ sys_cell_base, const int32_t *__restrict__ sys_brick_base,
     int nsys, int32_t M, int64_t nat, const int32_t *__restrict__ cell_start, const Rec *__restrict__ rec,
-    const int32_t *__restrict__ wbig, int32_t *__restrict__ pair_count, uint64_t *__restrict__ masks)
+    const int32_t *__restrict__ wbig, int32_t *__restrict__ pair_count, void *__restrict__ slots,
+    double *__restrict__ slotsR, int32_t cap, Status *__restrict__ status)
 {
     __shared__ BrickLDS<true> L;
     __shared__ double Csh[9];
     const PairsArgs A{sys, sys_cell_base, nsys, M, nat, cell_start, rec, wbig};
+    const SlotArgs O{slots, slotsR, cap, status};
     Brick B;
     brick_locate(A, sys_brick_base, B);
     if (threadIdx.x < 9) Csh[threadIdx.x] = B.sd->C[threadIdx.x];   // visible after brick_load's barrier
     const int wv = sgpr((int)(threadIdx.x >> 6));
     const bool lds = B.fast && brick_load<true>(A, B, L);
     if (!lds) {
-        for (int hc = wv; hc < B.nhome; hc += kBrickWaves) count_cell(A, *B.sd, brick_cell_id(B, hc), pair_count, masks);
+        for (int hc = wv; hc < B.nhome; hc += kBrickWaves)
+            count_cell<WANT_R, WIDE>(A, *B.sd, brick_cell_id(B, hc), pair_count, O);
         return;
     }
     const SysDev &sd = *B.sd;
     const double rc2 = sd.rc2;
     const int lane = lane_id();
-    constexpr double kFar = 1.0e200;   // invalid candidate: r^2 overflows to +inf, never a hit
     for (int hc = wv; hc < B.nhome; hc += kBrickWaves) {
         const HomeRows R = brick_home(B, L, hc);
         const int nha = R.h1 - R.h0;
         for (int hg = 0; hg < nha; hg += kWave) {
             const int nh = sgpr(min(kWave, nha - hg));
             const int32_t hj_lane = lane < nh ? L.j[R.h0 + hg + lane] : 0;
-            int cnt = 0;
-            int q = 0;
+            const int64_t p0 = (int64_t)R.pbase + hg;   // sorted index of home atom 0 of the group
+            int32_t cnt_lane = 0;                        // lane a: running count of home atom a
             for (int k0 = 0; k0 < R.total; k0 += kChunksPerPass * kWave) {
                 const int nch = sgpr(min(kChunksPerPass, (R.total - k0 + kWave - 1) / kWave));
                 double cxp[kChunksPerPass], cyp[kChunksPerPass], czp[kChunksPerPass];
-                // self key: j where the candidate's cell shift is zero, else -3
-                // (never a home index), so (key == j_i) is _is_self_interaction
-                int32_t key[kChunksPerPass], ctx[kChunksPerPass], cty[kChunksPerPass], ctz[kChunksPerPass];
-                uint64_t mine[kChunksPerPass];
+                // key: j where the candidate's cell shift is zero, else -3 (never
+                // a home index), so (key == j_i) is _is_self_interaction
+                int32_t cj[kChunksPerPass], key[kChunksPerPass];
+                uint32_t tpk[kChunksPerPass];   // packed s - w_j
                 double sx[kChunksPerPass], sy[kChunksPerPass], sz[kChunksPerPass];
 #pragma unroll
                 for (int g = 0; g < kChunksPerPass; ++g) {
-                    mine[g] = 0;
                     cxp[g] = cyp[g] = czp[g] = kFar;
                     sx[g] = sy[g] = sz[g] = 0.0;
+                    cj[g] = -1;
                     key[g] = -3;
-                    ctx[g] = cty[g] = ctz[g] = 0;
+                    tpk[g] = pack_wind(0, 0, 0);
                     if (g < nch) {
                         const int k = k0 + g * kWave + lane;
                         const bool v = k < R.total;
@@ -632,13 +564,15 @@ __global__ __launch_bounds__(kBrickThreads) void count_kernel(
                             cxp[g] = L.x[li];
                             cyp[g] = L.y[li];
                             czp[g] = L.z[li];
+                            cj[g] = L.j[li];
                             const uint32_t t = L.t[li];
-                            key[g] = (t & kZeroShiftBit) ? L.j[li] : -3;
-                            unpack_wind(t & ~kZeroShiftBit, ctx[g], cty[g], ctz[g]);
+                            key[g] = (t & kZeroShiftBit) ? cj[g] : -3;
+                            tpk[g] = t & ~kZeroShiftBit;
                         }
                     }
                 }
                 uint32_t wcache = 0xffffffffu;
+                int32_t wix = 0, wiy = 0, wiz = 0;
                 for (int a = 0; a < nh; ++a) {
                     const int ha = R.h0 + hg + a;   // uniform LDS address
                     const double hx = L.x[ha], hy = L.y[ha], hz = L.z[ha];
@@ -647,11 +581,20 @@ __global__ __launch_bounds__(kBrickThreads) void count_kernel(
                     if (ht != wcache) {
                         int32_t tx, ty, tz;
                         unpack_wind(ht, tx, ty, tz);
+                        wix = -tx; wiy = -ty; wiz = -tz;
 #pragma unroll
-                        for (int g = 0; g < kChunksPerPass; ++g)
-                            if (g < nch) cell_shift(Csh, ctx[g] - tx, cty[g] - ty, ctz[g] - tz, sx[g], sy[g], sz[g]);
+                        for (int g = 0; g < kChunksPerPass; ++g) {
+                            if (g < nch) {
+                                int32_t cx, cy, cz;
+                                unpack_wind(tpk[g], cx, cy, cz);
+                                cell_shift(Csh, cx + wix, cy + wiy, cz + wiz, sx[g], sy[g], sz[g]);
+                            }
+                        }
                         wcache = ht;
                     }
+                    const bool wzero = (wix | wiy | wiz) == 0;
+                    int32_t ca = __builtin_amdgcn_readlane(cnt_lane, a);
+                    const int64_t sbase = (p0 + a) * (int64_t)cap;
 #pragma unroll
                     for (int g = 0; g < kChunksPerPass; ++g) {
                         if (g < nch) {
@@ -661,180 +604,129 @@ __global__ __launch_bounds__(kBrickThreads) void count_kernel(
                             const double Rz = (czp[g] - hz) + sz[g];
                             const double r2 = (Rx * Rx + Ry * Ry) + Rz * Rz;
                             const uint64_t mask = __ballot((r2 < rc2) & (key[g] != hj));
-                            mine[g] = lane == a ? mask : mine[g];
+                            int32_t tx = 0, ty = 0, tz = 0;
+                            if (WIDE || !wzero) unpack_wind(tpk[g], tx, ty, tz);
+                            emit_hits<WANT_R, WIDE>(O, mask, sbase, ca, cj[g], tpk[g], tx, ty, tz, wix, wiy, wiz, wzero,
+                                                    Rx, Ry, Rz);
                         }
                     }
+                    cnt_lane = lane == a ? ca : cnt_lane;
                 }
-#pragma unroll
-                for (int g = 0; g < kChunksPerPass; ++g) {
-                    if (g < nch) {
-                        cnt += __popcll(mine[g]);
-                        if (q + g < kMaskChunks && lane < nh)
-                            masks[(int64_t)(q + g) * nat + R.pbase + hg + lane] = mine[g];
-                    }
-                }
-                q += nch;
             }
-            if (lane < nh) pair_count[sd.atom_base + hj_lane] = cnt;
+            if (lane < nh) pair_count[sd.atom_base + hj_lane] = cnt_lane;
         }
     }
 }
 
-template <bool WANT_R, typename TI>
-__global__ __launch_bounds__(kBrickThreads) void fill_kernel(
-    const SysDev *__restrict__ sys, const int32_t *__restrict__ sys_cell_base, const int32_t *__restrict__ sys_brick_base,
-    int nsys, int32_t M, int64_t nat, const int32_t *__restrict__ cell_start, const Rec *__restrict__ rec,
-    const int32_t *__restrict__ wbig, const uint64_t *__restrict__ masks, const int64_t *__restrict__ pair_off,
-    const Status *__restrict__ status, TI *__restrict__ i_out, TI *__restrict__ j_out, TI *__restrict__ S_out,
-    double *__restrict__ R_out)
+// Fill pass: atoms in their original order, one wave per 64 consecutive
+// atoms and one lane per pair, so the (i, j, S[, R]) stores of a wave cover
+// one contiguous range of the output arrays (fill_pairs_kernel!,
+// gpu_kernels.jl:159-180).
+template <bool WANT_R, bool WIDE, typename TI>
+__global__ __launch_bounds__(256) void fill_kernel(
+    const int64_t *__restrict__ sys_atom_base, int nsys, int64_t nat, const int32_t *__restrict__ pair_count,
+    const int64_t *__restrict__ pair_off, const int32_t *__restrict__ inv_perm, const void *__restrict__ slots,
+    const double *__restrict__ slotsR, const Status *__restrict__ status, int32_t cap, int64_t pair_capacity,
+    TI *__restrict__ i_out, TI *__restrict__ j_out, TI *__restrict__ S_out, double *__restrict__ R_out)
 {
-    __shared__ BrickLDS<WANT_R> L;
-    if (status->overflow) return;
-    const PairsArgs A{sys, sys_cell_base, nsys, M, nat, cell_start, rec, wbig};
-    Brick B;
-    brick_locate(A, sys_brick_base, B);
-    const int wv = sgpr((int)(threadIdx.x >> 6));
-    const bool lds = B.fast && brick_load<WANT_R>(A, B, L);
-    if (!lds) {
-        for (int hc = wv; hc < B.nhome; hc += kBrickWaves)
-            fill_cell<WANT_R, TI>(A, *B.sd, brick_cell_id(B, hc), masks, pair_off, i_out, j_out, S_out, R_out);
-        return;
-    }
-    const SysDev &sd = *B.sd;
-    const double *__restrict__ Cm = sd.C;
+    if (status->npairs > pair_capacity || status->max_count > cap || (!WIDE && status->wide)) return;
+    using E = SlotEntry<WIDE>;
     const int lane = lane_id();
-    for (int hc = wv; hc < B.nhome; hc += kBrickWaves) {
-        const HomeRows R = brick_home(B, L, hc);
-        const int nha = R.h1 - R.h0;
-        if (nha == 0) continue;
-        if ((R.total + kWave - 1) / kWave > kMaskChunks) {   // more chunks than stored masks
-            fill_cell<WANT_R, TI>(A, sd, brick_cell_id(B, hc), masks, pair_off, i_out, j_out, S_out, R_out);
-            continue;
+    const int64_t a0 = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(kWave - 1)));
+    if (a0 >= nat) return;
+    const int64_t a = a0 + lane;
+    const bool va = a < nat;
+    const int cnt = va ? pair_count[a] : 0;
+    const int64_t off = va ? pair_off[a] : 0;
+    const int64_t sbase = va ? (int64_t)inv_perm[a] * cap : 0;
+    int64_t il = a + 1;   // 1-based system-local index of atom a
+    if (nsys > 1 && va) il = a - sys_atom_base[find_sys(sys_atom_base, nsys, a)] + 1;
+    const int incl = wave_incl_scan(cnt);
+    const int tot = sgpr(__shfl(incl, kWave - 1, kWave));
+    const int ex = incl - cnt;
+    for (int h0 = 0; h0 < tot; h0 += kWave) {
+        const int L = h0 + lane;
+        int s = 0;   // atom of this pair: largest s with ex[s] <= L
+#pragma unroll
+        for (int step = 32; step >= 1; step >>= 1) {
+            const int e2 = __shfl(ex, s + step, kWave);
+            if (e2 <= L) s += step;
         }
-        for (int hg = 0; hg < nha; hg += kWave) {
-            const int nh = sgpr(min(kWave, nha - hg));
-            const int hl = R.h0 + hg + (lane < nh ? lane : 0);
-            const int32_t hj_l = L.j[hl];
-            int32_t wx = 0, wy = 0, wz = 0;   // w_i = -(t of the home atom)
-            {
-                int32_t tx, ty, tz;
-                unpack_wind(L.t[hl] & ~kZeroShiftBit, tx, ty, tz);
-                wx = -tx; wy = -ty; wz = -tz;
-            }
-            double hx = 0, hy = 0, hz = 0;
+        const int r = L - __shfl(ex, s, kWave);
+        const int64_t pos = (int64_t)shfl64((uint64_t)off, s) + r;
+        const int64_t sidx = (int64_t)shfl64((uint64_t)sbase, s) + r;
+        const int64_t is = (int64_t)shfl64((uint64_t)il, s);
+        if (L < tot) {
+            int32_t j, Sx, Sy, Sz;
+            E::get(reinterpret_cast<const typename E::T *>(slots)[sidx], j, Sx, Sy, Sz);
+            i_out[pos] = (TI)is;
+            j_out[pos] = (TI)(j + 1);
+            TI *S = S_out + 3 * pos;
+            S[0] = (TI)Sx;
+            S[1] = (TI)Sy;
+            S[2] = (TI)Sz;
             if (WANT_R) {
-                hx = L.x[hl];
-                hy = L.y[hl];
-                hz = L.z[hl];
-            }
-            int64_t cursor = lane < nh ? pair_off[sd.atom_base + hj_l] : 0;
-            uint64_t mk[kMaskChunks];
-            const int nq = (R.total + kWave - 1) / kWave;
-#pragma unroll
-            for (int q = 0; q < kMaskChunks; ++q)
-                mk[q] = (q < nq && lane < nh) ? masks[(int64_t)q * nat + R.pbase + hg + lane] : 0;
-            for (int q0 = 0; q0 < nq; q0 += kChunksPerPass) {
-                // one pass: chunks q0 .. q0+3, lanes = hits
-                uint64_t mine[kChunksPerPass];
-#pragma unroll
-                for (int g = 0; g < kChunksPerPass; ++g) {
-                    mine[g] = 0;
-#pragma unroll
-                    for (int q = 0; q < kMaskChunks; ++q)
-                        if (q == q0 + g) mine[g] = mk[q];
-                }
-                int pc = 0;
-#pragma unroll
-                for (int g = 0; g < kChunksPerPass; ++g) pc += __popcll(mine[g]);
-                const int incl = wave_incl_scan(pc);
-                const int tot = sgpr(__shfl(incl, kWave - 1, kWave));
-                const int exh = incl - pc;
-                for (int e0 = 0; e0 < tot; e0 += kWave) {
-                    const int Lh = e0 + lane;
-                    int a = 0;   // home atom: largest a with exh[a] <= Lh
-#pragma unroll
-                    for (int step = 32; step >= 1; step >>= 1) {
-                        const int e2 = __shfl(exh, a + step, kWave);
-                        if (e2 <= Lh) a += step;
-                    }
-                    const int rank = Lh - __shfl(exh, a, kWave);
-                    int r = rank, gsel = -1;
-                    uint64_t m = 0;
-#pragma unroll
-                    for (int g = 0; g < kChunksPerPass; ++g) {
-                        const uint64_t mg = shfl64(mine[g], a);
-                        const int pg = __popcll(mg);
-                        const bool here = (gsel < 0) & (r < pg);
-                        m = here ? mg : m;
-                        gsel = here ? g : gsel;
-                        r = (gsel < 0) ? r - pg : r;
-                    }
-                    const int k = (q0 + gsel) * kWave + nth_set_bit(m, r);
-                    const int32_t hj = __shfl(hj_l, a, kWave);
-                    const int32_t wix = __shfl(wx, a, kWave), wiy = __shfl(wy, a, kWave), wiz = __shfl(wz, a, kWave);
-                    const int64_t pos = (int64_t)shfl64((uint64_t)cursor, a) + rank;
-                    double hxa = 0, hya = 0, hza = 0;
-                    if (WANT_R) {
-                        hxa = shfl_d(hx, a);
-                        hya = shfl_d(hy, a);
-                        hza = shfl_d(hz, a);
-                    }
-                    const int li = row_index(R, Lh < tot ? k : 0);
-                    if (Lh < tot) {
-                        int32_t tx, ty, tz;
-                        unpack_wind(L.t[li] & ~kZeroShiftBit, tx, ty, tz);
-                        const int32_t Sx = tx + wix, Sy = ty + wiy, Sz = tz + wiz;
-                        i_out[pos] = (TI)(hj + 1);
-                        j_out[pos] = (TI)(L.j[li] + 1);
-                        TI *S = S_out + 3 * pos;
-                        S[0] = (TI)Sx;
-                        S[1] = (TI)Sy;
-                        S[2] = (TI)Sz;
-                        if (WANT_R) {   // the count pass's expression, bit for bit
-                            double cx, cy, cz;
-                            cell_shift(Cm, Sx, Sy, Sz, cx, cy, cz);
-                            double *Rp = R_out + 3 * pos;
-                            Rp[0] = (L.x[li] - hxa) + cx;
-                            Rp[1] = (L.y[li] - hya) + cy;
-                            Rp[2] = (L.z[li] - hza) + cz;
-                        }
-                    }
-                }
-                cursor += pc;
+                const double *Rs = slotsR + 3 * sidx;
+                double *Rd = R_out + 3 * pos;
+                Rd[0] = Rs[0];
+                Rd[1] = Rs[1];
+                Rd[2] = Rs[2];
             }
         }
     }
 }
+
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 
-void launch_count(const PipelineBufs &b, hipStream_t st)
+template <bool R, bool W>
+static void count_launch(const PipelineBufs &b, hipStream_t st)
 {
-    if (b.nat == 0 || b.nbricks == 0) return;
-    hipLaunchKernelGGL(count_kernel, dim3((unsigned)b.nbricks), dim3(kBrickThreads), 0, st, b.sys, b.sys_cell_base,
-                       b.sys_brick_base, b.nsys, b.M, b.nat, b.cell_start, b.rec, b.wbig, b.pair_count, b.masks);
+    hipLaunchKernelGGL((count_kernel<R, W>), dim3((unsigned)b.nbricks), dim3(kBrickThreads), 0, st, b.sys,
+                       b.sys_cell_base, b.sys_brick_base, b.nsys, b.M, b.nat, b.cell_start, b.rec, b.wbig,
+                       b.pair_count, b.slots, b.slotsR, b.slot_cap, b.status);
 }
 
-template <bool R, typename TI>
+void launch_count(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st)
+{
+    if (b.nat == 0 || b.nbricks == 0) return;
+    if (want_R) {
+        if (wide) count_launch<true, true>(b, st);
+        else count_launch<true, false>(b, st);
+    } else {
+        if (wide) count_launch<false, true>(b, st);
+        else count_launch<false, false>(b, st);
+    }
+}
+
+template <bool R, bool W, typename TI>
 static void fill_launch(const PipelineBufs &b, hipStream_t st)
 {
-    hipLaunchKernelGGL((fill_kernel<R, TI>), dim3((unsigned)b.nbricks), dim3(kBrickThreads), 0, st, b.sys,
-                       b.sys_cell_base, b.sys_brick_base, b.nsys, b.M, b.nat, b.cell_start, b.rec, b.wbig, b.masks,
-                       b.pair_off, b.status, (TI *)b.i_out, (TI *)b.j_out, (TI *)b.S_out, b.R_out);
+    const unsigned nblk = (unsigned)((b.nat + 255) / 256);
+    hipLaunchKernelGGL((fill_kernel<R, W, TI>), dim3(nblk), dim3(256), 0, st, b.sys_atom_base, b.nsys, b.nat,
+                       b.pair_count, b.pair_off, b.inv_perm, b.slots, b.slotsR, b.status, b.slot_cap, b.pair_capacity,
+                       (TI *)b.i_out, (TI *)b.j_out, (TI *)b.S_out, b.R_out);
 }
 
-void launch_fill(const PipelineBufs &b, bool want_R, hipStream_t st)
+template <typename TI>
+static void fill_dispatch(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st)
 {
-    if (b.nat == 0 || b.nbricks == 0) return;
-    if (b.ti_bytes == 8) {
-        if (want_R) fill_launch<true, int64_t>(b, st);
-        else fill_launch<false, int64_t>(b, st);
+    if (want_R) {
+        if (wide) fill_launch<true, true, TI>(b, st);
+        else fill_launch<true, false, TI>(b, st);
     } else {
-        if (want_R) fill_launch<true, int32_t>(b, st);
-        else fill_launch<false, int32_t>(b, st);
+        if (wide) fill_launch<false, true, TI>(b, st);
+        else fill_launch<false, false, TI>(b, st);
     }
+}
+
+void launch_fill(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st)
+{
+    if (b.nat == 0) return;
+    if (b.ti_bytes == 8) fill_dispatch<int64_t>(b, want_R, wide, st);
+    else fill_dispatch<int32_t>(b, want_R, wide, st);
 }
 
 }  // namespace nlh
