@@ -1,0 +1,222 @@
+"""Benchmark of the sort-based neighbour-list build on one MI355X.
+
+Headline (the reference's README benchmark, README.md:119-133 /
+scripts/benchmark.jl:57-151): pairs per second of a full build --
+cell list + pair materialisation -- of 100,000 atoms uniform-random in a cube
+at 0.05 atoms/A^3 (L = 125.99 A), cutoff 5 A, full pbc, Float64, with the
+positions already resident in HBM.
+
+Also reported on the same JSON line:
+  roofline      fill kernel (PairList writer) at 1,000,000 atoms, HBM-bound:
+                algorithmic bytes 20 P + 32 nat + 4 (M+1) per launch over its
+                mean duration from HIP events on the build stream
+  count_fp64    the distance-test kernel's FP64 rate (11 flop per pair test)
+  builds        1M-atom cube and the 1,000,188-atom sheared FCC slab
+  batched       4,096 small systems in one batched build vs one call per system
+  cpu_baseline  the CPU oracle (C restatement of the reference's sort-based
+                path) on the host cores, bounded sample
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import statistics
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "neighbourlists.jl_amd"))
+sys.path.insert(0, REPO)
+
+import numpy as np  # noqa: E402
+
+A4500_PAIRS_PER_S = 2.618e6 / 6.9e-3   # README.md:129: 100k atoms, 2.6M pairs, 6.9 ms on an RTX A4500
+PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6
+
+
+def _sync():
+    """Device-wide wait (the builds already wait on their own stream)."""
+    from nlhip import _lib
+
+    _lib.lib().nlh_device_synchronize()
+
+
+def _single(s, steps: int, warmup: int, profile: bool = False):
+    """Time `steps` full builds of one system (X resident on the device)."""
+    from nlhip import _lib
+
+    X = _lib.DeviceArray(np.ascontiguousarray(s.X, dtype=np.float64))   # resident in HBM
+    g = _lib.geometry(s.cell, s.cutoff, s.pbc, 4)
+    h = _lib.Handle()
+    n = C.c_int64()
+    L = _lib.lib()
+
+    def step():
+        _lib.check(L.nlh_neighbour_list(h.ptr, X.ptr, X.shape[0], C.byref(g), _lib.NLH_X_DEVICE, C.byref(n)))
+
+    for _ in range(warmup):
+        step()
+    _sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    _sync()
+    wall = (time.perf_counter() - t0) / steps
+    phases = None
+    if profile:   # separate, event-instrumented runs for the per-kernel durations
+        h.set_profiling(True)
+        acc = {}
+        reps = max(5, min(steps, 20))
+        for _ in range(reps):
+            step()
+            for k, v in h.phase_times().items():
+                acc.setdefault(k, []).append(v)
+        h.set_profiling(False)
+        phases = {k: statistics.mean(v) for k, v in acc.items()}
+    return {"npairs": int(n.value), "nat": int(X.shape[0]), "ms": wall * 1e3, "pairs_per_s": n.value / wall,
+            "ncells": int(g.ncells_total), "phases_ms": phases}
+
+
+def _roofline(r) -> dict:
+    P, nat, M = r["npairs"], r["nat"], r["ncells"]
+    b = 20 * P + 32 * nat + 4 * (M + 1)
+    t = r["phases_ms"]["fill"] * 1e-3
+    achieved = b / t / 1e9
+    traffic = None
+    tf = os.path.join(REPO, "profiles", "fill_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("fill_bytes_per_launch")
+    return {"kernel": "fill_kernel", "workload": f"cube {nat} atoms", "bound": "hbm", "achieved": round(achieved, 1),
+            "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+            "algorithmic_bytes": b, "fill_us": round(t * 1e6, 2)}
+
+
+def _count_rate(r, cand_per_atom: float) -> dict:
+    tests = r["nat"] * cand_per_atom
+    t = r["phases_ms"]["count"] * 1e-3
+    tflops = tests * 11 / t / 1e12
+    return {"kernel": "count_kernel", "bound": "fp64-vector", "achieved": round(tflops, 3), "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(tflops / PEAK_FP64_TFLOPS, 4), "count_us": round(t * 1e6, 2)}
+
+
+def _batched(nsys: int, steps: int):
+    from nlhip import _lib, workloads
+    from nlhip.batch import BatchBuilder, pack_systems
+
+    systems = workloads.batch_systems(nsys, seed=7)
+    X, offs = pack_systems(systems)
+    Xd = _lib.DeviceArray(X)
+    bb = BatchBuilder()
+    bb.run(systems, X_device=Xd, copy=False)   # warm-up (buffers, capacities)
+    _sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        P = bb.rebuild(X, X_device=Xd, copy=False)   # same systems, positions in HBM
+    _sync()
+    tb = (time.perf_counter() - t0) / steps
+    # plain path: the single-system call per system, X resident, each call waits
+    geoms = [_lib.geometry(s.cell, s.cutoff, s.pbc, 4) for s in systems]
+    h = _lib.Handle()
+    n = C.c_int64()
+    L = _lib.lib()
+    base = Xd.ptr
+
+    def plain():
+        tot = 0
+        for k, s in enumerate(systems):
+            _lib.check(L.nlh_neighbour_list(h.ptr, base + 24 * int(offs[k]), int(offs[k + 1] - offs[k]),
+                                            C.byref(geoms[k]), _lib.NLH_X_DEVICE, C.byref(n)))
+            tot += n.value
+        return tot
+
+    plain()
+    _sync()
+    t0 = time.perf_counter()
+    Pp = plain()
+    _sync()
+    tp = time.perf_counter() - t0
+    assert Pp == P, (Pp, P)
+    return {"nsys": nsys, "atoms": int(offs[-1]), "pairs": int(P), "batched_ms": round(tb * 1e3, 3),
+            "batched_systems_per_s": round(nsys / tb, 1), "plain_ms": round(tp * 1e3, 3),
+            "plain_systems_per_s": round(nsys / tp, 1), "speedup": round(tp / tb, 2)}
+
+
+def _cpu_baseline(s) -> dict:
+    import oracle
+
+    nt = min(16, os.cpu_count() or 1)
+    oracle.time_neighbour_list(s.X, s.cutoff, s.cell, s.pbc, nt)   # warm-up
+    ts = []
+    P = 0
+    for _ in range(5):
+        t0 = time.perf_counter()
+        P = oracle.time_neighbour_list(s.X, s.cutoff, s.cell, s.pbc, nt)
+        ts.append(time.perf_counter() - t0)
+    return {"value": round(P / min(ts), 1), "unit": "pairs/s", "cores": nt, "kind": "port",
+            "median_value": round(P / statistics.median(ts), 1),
+            "sample": f"5 full builds of the 100k-atom cube after 1 warm-up (min reported); C oracle, count and "
+                      f"fill over {nt} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--quick", action="store_true", help="headline only")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU; multi-GPU runs are not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    import nlhip
+    from nlhip import workloads
+
+    nlhip.init(0)
+    c1 = workloads.random_cubic(100_000, density=0.05, seed=1, cutoff=5.0)
+    head = _single(c1, args.steps, args.warmup, profile=True)
+    out = {
+        "metric": "pairs/sec at 100k atoms",
+        "value": round(head["pairs_per_s"], 1),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": round(head["pairs_per_s"] / A4500_PAIRS_PER_S, 2),
+        "baseline_source": "reference README.md:129 (RTX A4500, 100k atoms, 2.6M pairs, 6.9 ms => 3.79e8 pairs/s)",
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "cube 100,000 atoms, uniform-random (seed 1), density 0.05/A^3, L=125.99 A, "
+                               "cutoff 5.0 A, pbc=(T,T,T), Float64, Int32 indices",
+                   "npairs": head["npairs"], "ncells": head["ncells"]},
+        "phases_ms_100k": {k: round(v, 4) for k, v in head["phases_ms"].items()},
+    }
+    if not args.quick:
+        c1m = workloads.random_cubic(1_000_000, density=0.05, seed=2, cutoff=5.0)
+        big = _single(c1m, 10, 3, profile=True)
+        out["roofline"] = _roofline(big)
+        out["count_fp64"] = _count_rate(big, 27 * big["nat"] / big["ncells"])
+        slab = workloads.triclinic_fcc_slab()
+        sl = _single(slab, 10, 3, profile=True)
+        out["builds"] = {
+            "cube_1M": {"atoms": big["nat"], "pairs": big["npairs"], "ms": round(big["ms"], 3),
+                        "pairs_per_s": round(big["pairs_per_s"], 1),
+                        "phases_ms": {k: round(v, 4) for k, v in big["phases_ms"].items()}},
+            "fcc_slab_1M": {"atoms": sl["nat"], "pairs": sl["npairs"], "ms": round(sl["ms"], 3),
+                            "pairs_per_s": round(sl["pairs_per_s"], 1),
+                            "phases_ms": {k: round(v, 4) for k, v in sl["phases_ms"].items()}},
+        }
+        out["batched"] = _batched(4096, 5)
+        out["cpu_baseline"] = _cpu_baseline(c1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

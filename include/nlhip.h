@@ -159,6 +159,14 @@ int nlh_batch_rebuild(nlh_handle *h, const double *X_batch, uint32_t flags, int6
 int nlh_set_profiling(nlh_handle *h, int enable);
 int nlh_phase_times(nlh_handle *h, float ms[5]);
 
+/* Device memory of the library's own HIP runtime, for callers that keep
+ * positions resident without another GPU runtime (no reference
+ * counterpart: Julia arrays come from AMDGPU.jl). */
+int nlh_device_malloc(uint64_t bytes, void **ptr);
+int nlh_device_free(void *ptr);
+int nlh_memcpy(void *dst, const void *src, uint64_t bytes); /* any direction */
+int nlh_device_synchronize(void);
+
 /* Library self-description for the host bindings. */
 int nlh_version(int32_t *major, int32_t *minor);
 

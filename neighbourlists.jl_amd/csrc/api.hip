@@ -639,6 +639,40 @@ __global__ void neighbours_one_kernel(nlh_celllist_dev<TI> c, int64_t i, int64_t
 
 extern "C" {
 
+int nlh_device_malloc(uint64_t bytes, void **ptr)
+{
+    return guarded([&] {
+        if (!ptr) throw Error(NLH_EINVAL, "ptr is NULL");
+        HIPC(hipSetDevice(g_device));
+        *ptr = nullptr;
+        HIPC(hipMalloc(ptr, bytes ? bytes : 16));
+    });
+}
+
+int nlh_device_free(void *ptr)
+{
+    return guarded([&] {
+        if (ptr) HIPC(hipFree(ptr));
+    });
+}
+
+int nlh_memcpy(void *dst, const void *src, uint64_t bytes)
+{
+    return guarded([&] {
+        if (bytes == 0) return;
+        if (!dst || !src) throw Error(NLH_EINVAL, "NULL pointer");
+        HIPC(hipMemcpy(dst, src, bytes, hipMemcpyDefault));
+    });
+}
+
+int nlh_device_synchronize(void)
+{
+    return guarded([&] {
+        HIPC(hipSetDevice(g_device));
+        HIPC(hipDeviceSynchronize());
+    });
+}
+
 int nlh_version(int32_t *major, int32_t *minor)
 {
     if (major) *major = 0;

@@ -99,7 +99,8 @@ EXPORTS = [
     "nlh_geometry_init", "nlh_build_cell_list", "nlh_materialize", "nlh_neighbour_list",
     "nlh_pairlist_device", "nlh_celllist_device", "nlh_copy_pairlist", "nlh_copy_celllist",
     "nlh_neighbours", "nlh_batch_neighbour_list", "nlh_batch_pair_offsets", "nlh_batch_rebuild",
-    "nlh_set_profiling", "nlh_phase_times", "nlh_version",
+    "nlh_set_profiling", "nlh_phase_times", "nlh_device_malloc", "nlh_device_free", "nlh_memcpy", "nlh_device_synchronize",
+    "nlh_version",
 ]
 
 
@@ -126,6 +127,10 @@ def _declare(L):
     L.nlh_batch_rebuild.argtypes = [hp, vp, C.c_uint32, i64p]
     L.nlh_set_profiling.argtypes = [hp, C.c_int]
     L.nlh_phase_times.argtypes = [hp, C.POINTER(C.c_float)]
+    L.nlh_device_malloc.argtypes = [C.c_uint64, C.POINTER(C.c_void_p)]
+    L.nlh_device_free.argtypes = [C.c_void_p]
+    L.nlh_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]
+    L.nlh_device_synchronize.argtypes = []
     L.nlh_version.argtypes = [C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     for name in EXPORTS:
         if name != "nlh_last_error":
@@ -138,6 +143,12 @@ def lib():
     if _lib is None:
         with _lock:
             if _lib is None:
+                # torch ships its own HIP/HSA runtimes: load them first so a
+                # later torch.cuda initialisation still sees the device
+                try:
+                    import torch  # noqa: F401
+                except ImportError:
+                    pass
                 if not os.path.exists(LIB_PATH):
                     raise RuntimeError(
                         f"libnlhip.so not found at {LIB_PATH}; build it first "
@@ -254,3 +265,42 @@ class Handle:
         ms = (C.c_float * 5)()
         check(lib().nlh_phase_times(self._h, ms), "phase_times")
         return dict(zip(("cells", "count", "scan", "fill", "total"), [float(v) for v in ms]))
+
+
+class DeviceArray:
+    """Device memory owned through libnlhip's own HIP runtime (float64 or
+    integer arrays kept resident in HBM)."""
+
+    def __init__(self, host: np.ndarray):
+        if not _inited:
+            init(0)
+        self.shape, self.dtype = host.shape, host.dtype
+        self.nbytes = int(host.nbytes)
+        p = C.c_void_p()
+        check(lib().nlh_device_malloc(max(self.nbytes, 16), C.byref(p)), "device_malloc")
+        self.ptr = p.value
+        self.upload(host)
+
+    def upload(self, host: np.ndarray):
+        h = np.ascontiguousarray(host, dtype=self.dtype)
+        if h.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        if self.nbytes:
+            check(lib().nlh_memcpy(self.ptr, h.ctypes.data, self.nbytes), "memcpy")
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        if self.nbytes:
+            check(lib().nlh_memcpy(out.ctypes.data, self.ptr, self.nbytes), "memcpy")
+        return out
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib().nlh_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -47,6 +47,11 @@ class BatchResult:
                               self.S[a:b], first, R)
 
 
+def _device_ptr(X_device) -> int:
+    """Device pointer of a DeviceArray or a torch CUDA tensor."""
+    return X_device.ptr if hasattr(X_device, "ptr") else X_device.data_ptr()
+
+
 def pack_systems(systems):
     """-> (X concatenated (nat,3), atom_offsets (nsys+1,))"""
     nat = np.array([s.X.shape[0] for s in systems], dtype=np.int64)
@@ -75,8 +80,9 @@ class BatchBuilder:
         return arr
 
     def run(self, systems, *, want_R: bool = False, X_device=None, copy: bool = True) -> BatchResult | int:
-        """Build all systems.  With X_device (a torch CUDA float64 tensor of
-        the concatenated positions) the positions are read from HBM.  With
+        """Build all systems.  With X_device (a DeviceArray or torch CUDA
+        float64 tensor of the concatenated positions) the positions are read
+        from HBM.  With
         copy=False only the pair count is returned (results stay on the
         device)."""
         X, offs = pack_systems(systems)
@@ -85,7 +91,7 @@ class BatchBuilder:
         ptr = X.ctypes.data
         if X_device is not None:
             flags |= NLH_X_DEVICE
-            ptr = X_device.data_ptr()
+            ptr = _device_ptr(X_device)
         n = C.c_int64()
         check(lib().nlh_batch_neighbour_list(self.handle.ptr, len(systems), ptr,
                                              offs.ctypes.data_as(C.POINTER(C.c_int64)), garr, flags, C.byref(n)),
@@ -102,7 +108,7 @@ class BatchBuilder:
         ptr = X.ctypes.data
         if X_device is not None:
             flags |= NLH_X_DEVICE
-            ptr = X_device.data_ptr()
+            ptr = _device_ptr(X_device)
         n = C.c_int64()
         check(lib().nlh_batch_rebuild(self.handle.ptr, ptr, flags, C.byref(n)), "batch_rebuild")
         if not copy:
