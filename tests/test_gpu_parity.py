@@ -6,6 +6,9 @@ the reference's evaluation order) must equal the oracle's element for
 element -- a stronger check than the reference's own set comparison
 (test/test_utils.jl:127-131).
 """
+import glob
+import os
+
 import numpy as np
 import pytest
 
@@ -177,3 +180,24 @@ def test_bricks_dense_triclinic():
     s = workloads.random_in_cell(4000, C, seed=62)
     _check(s.X, 4.0, C, (True, True, True))
     _check(s.X, 4.0, C, (True, False, True))
+
+
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "*.npz")))
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[:-4] for p in GOLDEN])
+def test_golden_fixture(path):
+    """Every committed fixture, bit for bit (pairs, R and the cell list)."""
+    with np.load(path, allow_pickle=False) as z:
+        f = {k: z[k] for k in z.files}
+    X, C, rc = f["X"], f["cell"], float(f["cutoff"])
+    pbc = tuple(bool(b) for b in f["pbc"])
+    print(f"golden {os.path.basename(path)} nat={X.shape[0]}", flush=True)
+    nl = nlhip.neighbour_list(X, rc, C, pbc, want_R=True)
+    assert nl.npairs() == f["i"].shape[0]
+    for k in ("i", "j", "S", "first"):
+        assert np.array_equal(getattr(nl, k), f[k]), k
+    assert np.array_equal(np.asarray(nl.R).view(np.int64), f["R"].view(np.int64))
+    cl = nlhip.build_cell_list(X, rc, C, pbc)
+    for k in ("perm", "cell_id", "cell_offsets"):
+        assert np.array_equal(getattr(cl, k), f[k]), k
