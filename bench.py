@@ -162,12 +162,46 @@ def _cpu_baseline(s) -> dict:
                       f"fill over {nt} threads"}
 
 
+def _copy_calibration(nbytes: int, reps: int) -> dict:
+    """Device-to-device copies of a known size: the reference point that turns
+    the HBM byte counters of a profiler run into bytes."""
+    from nlhip import _lib
+
+    a = _lib.DeviceArray(np.zeros(nbytes // 8))
+    b = _lib.DeviceArray(np.zeros(nbytes // 8))
+    for _ in range(reps):
+        _lib.check(_lib.lib().nlh_memcpy(b.ptr, a.ptr, nbytes), "copy")
+    _sync()
+    a.free()
+    b.free()
+    return {"copy_bytes": nbytes, "reps": reps}
+
+
+def _only(name: str, steps: int, warmup: int) -> dict:
+    """One workload of the full run, alone (for profiler runs)."""
+    from nlhip import workloads
+
+    if name == "cube_100k":
+        return _single(workloads.random_cubic(100_000, density=0.05, seed=1, cutoff=5.0), steps, warmup)
+    if name == "cube_1M":
+        return _single(workloads.random_cubic(1_000_000, density=0.05, seed=2, cutoff=5.0), steps, warmup)
+    if name == "fcc_slab_1M":
+        return _single(workloads.triclinic_fcc_slab(), steps, warmup)
+    if name == "batched":
+        return _batched(4096, steps)
+    if name == "copy":
+        return _copy_calibration(512 << 20, steps)
+    raise SystemExit(f"unknown workload {name}")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--quick", action="store_true", help="headline only")
+    ap.add_argument("--only", choices=["cube_100k", "cube_1M", "fcc_slab_1M", "batched", "copy"],
+                    help="run one workload alone (profiler runs); prints its own JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU; multi-GPU runs are not part of this build", file=sys.stderr)
@@ -177,6 +211,10 @@ def main():
     from nlhip import workloads
 
     nlhip.init(0)
+    if args.only:
+        r = _only(args.only, args.steps, args.warmup)
+        print(json.dumps({"workload": args.only, **{k: v for k, v in r.items() if k != "phases_ms"}}), flush=True)
+        return
     c1 = workloads.random_cubic(100_000, density=0.05, seed=1, cutoff=5.0)
     head = _single(c1, args.steps, args.warmup, profile=True)
     out = {

@@ -38,9 +38,13 @@ def hipcc() -> str:
     return "hipcc"
 
 
-def build(verbose: bool = False) -> str:
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = OUT + ".tmp"
+USER_KERNEL_DIR = os.path.join(REPO, "tests", "user_kernel")
+USER_KERNEL_SRC = os.path.join(USER_KERNEL_DIR, "user_kernel.hip")
+USER_KERNEL_OUT = os.path.join(USER_KERNEL_DIR, "libnlh_user_kernel.so")
+
+
+def _compile(srcs, out, verbose):
+    tmp = out + ".tmp"
     cmd = [hipcc()] + HIPCC_FLAGS + ["-I", INCLUDE, "-o", tmp] + srcs
     r = subprocess.run(cmd, capture_output=True, text=True)
     noise = "is not a recognized feature for this target"
@@ -49,8 +53,17 @@ def build(verbose: bool = False) -> str:
         raise RuntimeError("hipcc failed:\n" + " ".join(cmd) + "\n" + err)
     if verbose and err.strip():
         print(err)
-    os.replace(tmp, OUT)
-    return OUT
+    os.replace(tmp, out)
+    return out
+
+
+def build(verbose: bool = False) -> str:
+    """libnlhip.so, then the test-only user kernel that exercises the
+    device-side header include/nlhip_device.h."""
+    out = _compile([os.path.join(CSRC, s) for s in SOURCES], OUT, verbose)
+    if os.path.exists(USER_KERNEL_SRC):
+        _compile([USER_KERNEL_SRC], USER_KERNEL_OUT, verbose)
+    return out
 
 
 if __name__ == "__main__":

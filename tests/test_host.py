@@ -130,3 +130,21 @@ def test_no_cpu_fallback_without_device():
     import nlhip
     with pytest.raises(_lib.NLHError):
         nlhip.neighbour_list(np.zeros((2, 3)), 1.0, np.eye(3) * 5, (True,) * 3)
+
+
+def test_julia_shim_binds_declared_symbols():
+    """The Julia ccall shim binds only entry points the header declares, and
+    its geometry struct has the C layout (no Julia here: checked as text)."""
+    jl = open(os.path.join(REPO, "neighbourlists.jl_amd", "julia", "NeighbourListsHIP.jl")).read()
+    called = set(re.findall(r"ccall\(\(:(nlh_[a-z0-9_]+),", jl))
+    assert called and called <= set(_declared())
+    assert {"nlh_geometry_init", "nlh_build_cell_list", "nlh_materialize", "nlh_copy_pairlist",
+            "nlh_copy_celllist", "nlh_neighbours", "nlh_batch_neighbour_list"} <= called
+    sizes = {"Float64": 8, "Int64": 8, "UInt8": 1, "Int32": 4}
+    body = jl[jl.index("struct NLHGeometry"):]
+    body = body[:body.index("\nend")]
+    total = 0
+    for name, typ in re.findall(r"^\s+(\w+)::(.+)$", body, flags=re.M):
+        m = re.match(r"NTuple\{(\d+),(\w+)\}", typ)
+        total += int(m.group(1)) * sizes[m.group(2)] if m else sizes[typ.strip()]
+    assert total == C.sizeof(_lib.Geometry) == 248
