@@ -46,6 +46,7 @@ typedef enum {
 #define NLH_X_DEVICE 0x1u  /* X (and X_batch) is a device pointer             */
 #define NLH_WANT_R 0x2u    /* also write R = X[j] - X[i] + C' S per pair      */
 #define NLH_NO_SYNC 0x4u   /* do not wait for the stream (nlh_synchronize)    */
+#define NLH_NO_GRAPH 0x8u  /* nlh_batch_rebuild: run the stages one by one    */
 
 /* Geometry of one system: the outputs of analyze_cell plus the neighbour
  * reach and cutoff^2 (src/cell_list.jl:152-170, 655-659;
@@ -146,9 +147,36 @@ int nlh_neighbours(nlh_handle *h, int32_t sys, int64_t i, int64_t cap, void *j, 
 int nlh_batch_neighbour_list(nlh_handle *h, int32_t nsys, const double *X_batch, const int64_t *atom_offsets,
                              const nlh_geometry *geoms, uint32_t flags, int64_t *total_pairs);
 int nlh_batch_pair_offsets(nlh_handle *h, int64_t *pair_offsets /* nsys + 1, host */);
-/* Replay the last batch's captured pipeline (hipGraph) on new positions of
- * the same systems (same atom counts and geometries). */
+/* Rebuild the last batch (or single system) on new positions of the same
+ * systems (same atom counts and geometries).  The first call captures the
+ * whole pipeline (status clear, binning, sort, cell offsets, pair count,
+ * pair scan, fill) as one HIP graph on the handle's stream; later calls
+ * replay it while buffers, capacities, flags and the position pointer stay
+ * the same (host positions are copied into the handle's own buffer first).
+ * A replay that outgrows the pair or slot capacity is redone on the stage
+ * path, which grows them.  NLH_NO_GRAPH forces the stage path.  The
+ * reference has no rebuild entry: a rebuild there is a fresh
+ * neighbour_list (src/cell_list.jl:897-916). */
 int nlh_batch_rebuild(nlh_handle *h, const double *X_batch, uint32_t flags, int64_t *total_pairs);
+
+/* Launch configuration of the build.  brick_atoms: expected atoms (halo
+ * included) of one count-pass brick, which sets the brick shape of each
+ * system (64..2048; the LDS holds 2048).  fill_block: threads per fill
+ * workgroup, a multiple of 64 up to 1024 (each wave writes the pairs of
+ * 64 consecutive atoms).  Every configuration gives identical results; no
+ * reference counterpart (KernelAbstractions picks workgroup sizes itself). */
+typedef struct {
+    int32_t brick_atoms;
+    int32_t fill_block;
+} nlh_tuning;
+int nlh_set_tuning(nlh_handle *h, const nlh_tuning *t);
+int nlh_get_tuning(nlh_handle *h, nlh_tuning *t);
+/* Time the current batch (set up by a previous build on this handle) under
+ * each candidate configuration (brick_atoms in {512, 1024, 1536} x
+ * fill_block in {128, 256, 512}, reps timed rebuilds each, HIP events on the
+ * handle's stream), keep the fastest and return it in *best.  Leaves the
+ * handle holding a build of X_batch under the winning configuration. */
+int nlh_autotune(nlh_handle *h, const double *X_batch, uint32_t flags, int32_t reps, nlh_tuning *best);
 
 /* Per-phase timing of the last build on this handle, from HIP events on
  * the handle's stream (enabled by nlh_set_profiling): ms[0] cell list (bin,

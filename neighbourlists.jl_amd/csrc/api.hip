@@ -118,14 +118,19 @@ struct nlh_handle {
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     float phase_ms[5] = {0, 0, 0, 0, 0};
     bool ev_build = false, ev_pairs = false;
-    // captured batch pipeline
+    // captured batch pipeline (nlh_batch_rebuild) and the buffers it was captured on
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
     bool graph_want_R = false;
+    nlh::PipelineBufs graph_bufs;
+    int64_t graph_launches = 0;
+    // launch configuration (nlh_set_tuning / nlh_autotune)
+    nlh_tuning tune = {nlh::kBrickAtomBudget, 256};
 
     nlh::PipelineBufs bufs() const
     {
-        nlh::PipelineBufs b{};
+        nlh::PipelineBufs b;
+        std::memset(&b, 0, sizeof b);   // padding too: captured configurations are compared bytewise
         b.X = X_dev;
         b.sys = sys.as<nlh::SysDev>();
         b.sys_atom_base = atom_base.as<int64_t>();
@@ -161,6 +166,7 @@ struct nlh_handle {
         b.pair_capacity = pair_capacity;
         b.status = status.as<nlh::Status>();
         b.ti_bytes = ti_bytes;
+        b.fill_block = tune.fill_block;
         return b;
     }
 };
@@ -329,7 +335,7 @@ void drop_graph(nlh_handle *h)
 // Pair-search brick of a system: the largest of a fixed ladder of shapes
 // whose halo is expected to fit the LDS budget at this system's mean atom
 // density per cell (clipped to the cell grid).
-void choose_brick(const nlh_geometry &g, int64_t nat, int32_t out[3])
+void choose_brick(const nlh_geometry &g, int64_t nat, int32_t budget, int32_t out[3])
 {
     static const int32_t ladder[][3] = {{4, 4, 4}, {4, 4, 2}, {4, 2, 2}, {2, 2, 2}, {2, 2, 1}, {2, 1, 1}, {1, 1, 1}};
     const double per_cell = g.ncells_total > 0 ? (double)nat / (double)g.ncells_total : 0.0;
@@ -340,12 +346,55 @@ void choose_brick(const nlh_geometry &g, int64_t nat, int32_t out[3])
             d[k] = (int32_t)std::min<int64_t>(b[k], g.ncells[k]);
             halo *= (double)(d[k] + 2);
         }
-        if (halo * per_cell * 1.25 <= (double)nlh::kBrickAtomBudget && halo <= 512.0) {
+        if (halo * per_cell * 1.25 <= (double)budget && halo <= 512.0) {
             std::memcpy(out, d, sizeof d);
             return;
         }
     }
     out[0] = out[1] = out[2] = 1;
+}
+
+// Pair-search bricks of every system for the current brick budget: brick
+// shape per system and the first brick of each system (host tables; the
+// caller uploads them).
+void assign_bricks(nlh_handle *h)
+{
+    int64_t bricks = 0;
+    const int nsys = (int)h->sys_host.size();
+    h->brick_base_host.resize(nsys + 1);
+    for (int s = 0; s < nsys; ++s) {
+        nlh::SysDev &sd = h->sys_host[s];
+        choose_brick(h->geoms[s], sd.nat, h->tune.brick_atoms, sd.brick);
+        sd.brick_base = (int32_t)bricks;
+        h->brick_base_host[s] = (int32_t)bricks;
+        bricks += (int64_t)((sd.n[0] + sd.brick[0] - 1) / sd.brick[0]) * ((sd.n[1] + sd.brick[1] - 1) / sd.brick[1]) *
+                  ((sd.n[2] + sd.brick[2] - 1) / sd.brick[2]);
+        if (bricks > INT32_MAX - 1) throw Error(NLH_EOVERFLOW, "too many pair-search bricks in one batch");
+    }
+    h->brick_base_host[nsys] = (int32_t)bricks;
+    h->nbricks = bricks;
+}
+
+// Upload the system tables (sys, atom/cell/brick bases) through pinned
+// staging.  Waits for earlier work on the stream first: staging is reused.
+void upload_tables(nlh_handle *h)
+{
+    const int nsys = (int)h->sys_host.size();
+    const size_t sys_bytes = nsys * sizeof(nlh::SysDev);
+    const size_t ab_bytes = (nsys + 1) * sizeof(int64_t);
+    const size_t cb_bytes = (nsys + 1) * sizeof(int32_t);
+    HIPC(hipStreamSynchronize(h->stream));
+    ensure_staging(h, sys_bytes + ab_bytes + 2 * cb_bytes + 64);
+    char *st = static_cast<char *>(h->staging);
+    std::memcpy(st, h->sys_host.data(), sys_bytes);
+    std::memcpy(st + sys_bytes, h->atom_base_host.data(), ab_bytes);
+    std::memcpy(st + sys_bytes + ab_bytes, h->cell_base_host.data(), cb_bytes);
+    std::memcpy(st + sys_bytes + ab_bytes + cb_bytes, h->brick_base_host.data(), cb_bytes);
+    HIPC(hipMemcpyAsync(h->sys.p, st, sys_bytes, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(h->atom_base.p, st + sys_bytes, ab_bytes, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(h->cell_base.p, st + sys_bytes + ab_bytes, cb_bytes, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipMemcpyAsync(h->brick_base.p, st + sys_bytes + ab_bytes + cb_bytes, cb_bytes, hipMemcpyHostToDevice,
+                        h->stream));
 }
 
 // Set up the system tables of a batch (a single build is nsys == 1) and
@@ -359,7 +408,7 @@ void setup_batch(nlh_handle *h, int32_t nsys, const int64_t *atom_offsets, const
     h->cell_base_host.resize(nsys + 1);
     h->brick_base_host.resize(nsys + 1);
     h->geoms.assign(geoms, geoms + nsys);
-    int64_t cells = 0, bricks = 0;
+    int64_t cells = 0;
     if (atom_offsets[0] != 0) throw Error(NLH_EINVAL, "atom_offsets[0] must be 0");
     for (int s = 0; s < nsys; ++s) {
         validate_geometry(geoms[s]);
@@ -368,16 +417,8 @@ void setup_batch(nlh_handle *h, int32_t nsys, const int64_t *atom_offsets, const
         if (na < 0) throw Error(NLH_EINVAL, "atom_offsets must be non-decreasing");
         if (ti == 4 && na >= INT32_MAX) throw Error(NLH_EOVERFLOW, "too many atoms for Int32 indices");
         h->sys_host[s] = make_sys(geoms[s], na, atom_offsets[s], (int32_t)cells, atom_offsets[s] + s);
-        choose_brick(geoms[s], na, h->sys_host[s].brick);
-        h->sys_host[s].brick_base = (int32_t)bricks;
         h->atom_base_host[s] = atom_offsets[s];
         h->cell_base_host[s] = (int32_t)cells;
-        h->brick_base_host[s] = (int32_t)bricks;
-        {
-            const nlh::SysDev &sd = h->sys_host[s];
-            bricks += (int64_t)((sd.n[0] + sd.brick[0] - 1) / sd.brick[0]) * ((sd.n[1] + sd.brick[1] - 1) / sd.brick[1]) *
-                      ((sd.n[2] + sd.brick[2] - 1) / sd.brick[2]);
-        }
         cells += geoms[s].ncells_total;
         if (cells > INT32_MAX - 1) throw Error(NLH_EOVERFLOW, "more than 2^31-1 cells in one batch");
     }
@@ -385,9 +426,7 @@ void setup_batch(nlh_handle *h, int32_t nsys, const int64_t *atom_offsets, const
     if (nat > INT32_MAX - 1) throw Error(NLH_EOVERFLOW, "more than 2^31-1 atoms in one build");
     h->atom_base_host[nsys] = nat;
     h->cell_base_host[nsys] = (int32_t)cells;
-    h->brick_base_host[nsys] = (int32_t)bricks;
-    if (bricks > INT32_MAX - 1) throw Error(NLH_EOVERFLOW, "too many pair-search bricks in one batch");
-    h->nbricks = bricks;
+    assign_bricks(h);
     h->nsys = nsys;
     h->nat = nat;
     h->M = (int32_t)cells;
@@ -413,20 +452,7 @@ void setup_batch(nlh_handle *h, int32_t nsys, const int64_t *atom_offsets, const
     h->partials.ensure((std::max(M1, n1) / nlh::kScanTile + 2) * sizeof(int64_t));
     h->status.ensure(sizeof(nlh::Status));
 
-    const size_t sys_bytes = nsys * sizeof(nlh::SysDev);
-    const size_t ab_bytes = (nsys + 1) * sizeof(int64_t);
-    const size_t cb_bytes = (nsys + 1) * sizeof(int32_t);
-    ensure_staging(h, sys_bytes + ab_bytes + 2 * cb_bytes + 64);
-    char *st = static_cast<char *>(h->staging);
-    std::memcpy(st, h->sys_host.data(), sys_bytes);
-    std::memcpy(st + sys_bytes, h->atom_base_host.data(), ab_bytes);
-    std::memcpy(st + sys_bytes + ab_bytes, h->cell_base_host.data(), cb_bytes);
-    std::memcpy(st + sys_bytes + ab_bytes + cb_bytes, h->brick_base_host.data(), cb_bytes);
-    HIPC(hipMemcpyAsync(h->sys.p, st, sys_bytes, hipMemcpyHostToDevice, h->stream));
-    HIPC(hipMemcpyAsync(h->atom_base.p, st + sys_bytes, ab_bytes, hipMemcpyHostToDevice, h->stream));
-    HIPC(hipMemcpyAsync(h->cell_base.p, st + sys_bytes + ab_bytes, cb_bytes, hipMemcpyHostToDevice, h->stream));
-    HIPC(hipMemcpyAsync(h->brick_base.p, st + sys_bytes + ab_bytes + cb_bytes, cb_bytes, hipMemcpyHostToDevice,
-                        h->stream));
+    upload_tables(h);
 }
 
 void upload_positions(nlh_handle *h, const double *X, uint32_t flags)
@@ -443,9 +469,7 @@ void upload_positions(nlh_handle *h, const double *X, uint32_t flags)
 
 void reset_status(nlh_handle *h)
 {
-    // Status is cleared by a tiny copy from pinned memory (no memset node).
-    std::memset(h->status_host, 0, sizeof(nlh::Status));
-    HIPC(hipMemcpyAsync(h->status.p, h->status_host, sizeof(nlh::Status), hipMemcpyHostToDevice, h->stream));
+    nlh::launch_clear_status(h->status.as<nlh::Status>(), h->stream);
 }
 
 void mark(nlh_handle *h, int k)
@@ -542,6 +566,25 @@ void check_positions(nlh_handle *h)
     }
 }
 
+void finish_pairs(nlh_handle *h, bool want_R);
+
+// Host-side bounds check of everything the count and fill kernels write:
+// slots of slot_cap entries per atom (and their R), pair outputs of
+// pair_capacity entries (the fill skips itself beyond that), per-atom
+// counts and offsets.
+void check_pair_buffers(const nlh_handle *h, bool want_R)
+{
+    const size_t n1 = (size_t)std::max<int64_t>(h->nat, 1);
+    const size_t ent = (size_t)h->slot_cap * n1;
+    const size_t ti = (size_t)h->ti_bytes, cap = (size_t)std::max<int64_t>(h->pair_capacity, 0);
+    bool ok = h->slot_cap > 0 && h->slots.p && h->slots.cap >= ent * (h->wide ? 16 : 8);
+    ok = ok && h->pair_count.cap >= n1 * 4 && h->pair_off.cap >= (n1 + 1) * 8 && h->inv_perm.cap >= n1 * 4;
+    ok = ok && h->first.cap >= (n1 + h->nsys) * ti && h->sys_pair_base.cap >= (size_t)(h->nsys + 1) * 8;
+    ok = ok && h->i.cap >= cap * ti && h->j.cap >= cap * ti && h->S.cap >= cap * 3 * ti;
+    if (want_R) ok = ok && h->slotsR.p && h->slotsR.cap >= ent * 24 && h->R.p && h->R.cap >= cap * 24;
+    if (!ok) throw Error(NLH_EDEVICE, "internal: pair-stage buffers are smaller than the launch needs");
+}
+
 // Pair stages.  Returns after the wait, with npairs known.
 void materialize(nlh_handle *h, uint32_t flags)
 {
@@ -567,6 +610,7 @@ void materialize(nlh_handle *h, uint32_t flags)
     }
     ensure_pair_outputs(h, std::max(h->pair_capacity, estimate_pairs(h)), want_R);
     ensure_slots(h, std::max(h->slot_cap, estimate_slot_cap(h)), want_R);
+    check_pair_buffers(h, want_R);
     nlh::PipelineBufs b = h->bufs();
     mark(h, 1);
     nlh::launch_count(b, want_R, h->wide, h->stream);
@@ -587,6 +631,7 @@ void materialize(nlh_handle *h, uint32_t flags)
         ensure_pair_outputs(h, std::max<int64_t>(st.npairs, 1), want_R);
         h->slots.release();
         ensure_slots(h, (int32_t)((std::max(st.max_count, 1) + 7) / 8) * 8, want_R);
+        check_pair_buffers(h, want_R);
         b = h->bufs();
         nlh::launch_count(b, want_R, h->wide, h->stream);
         nlh::launch_fill(b, want_R, h->wide, h->stream);
@@ -595,6 +640,12 @@ void materialize(nlh_handle *h, uint32_t flags)
         if (h->status_host->npairs > h->pair_capacity || h->status_host->max_count > h->slot_cap)
             throw Error(NLH_EDEVICE, "pair capacity still exceeded after growth");
     }
+    finish_pairs(h, want_R);
+}
+
+// Bookkeeping after a successful pair pass (status already on the host).
+void finish_pairs(nlh_handle *h, bool want_R)
+{
     h->npairs = h->status_host->npairs;
     h->have_R = want_R;
     // per-system pair offsets
@@ -610,6 +661,87 @@ void materialize(nlh_handle *h, uint32_t flags)
             if (h->pair_offsets_host[s + 1] - h->pair_offsets_host[s] >= INT32_MAX)
                 throw Error(NLH_EOVERFLOW, "pair count exceeds typemax(Int32); use int_type=Int64");
     h->materialized = true;
+}
+
+// The whole rebuild of the current batch -- status clear, binning, sort,
+// offsets, count, pair scan, fill -- captured once as a HIP graph on the
+// handle's stream.  Buffers, capacities and positions are baked in; the
+// capture is redone when any of them changes.  No memset nodes: the status
+// is cleared by a kernel.
+void capture_graph(nlh_handle *h, bool want_R)
+{
+    drop_graph(h);
+    const nlh::PipelineBufs b = h->bufs();
+    HIPC(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    nlh::launch_clear_status(b.status, h->stream);
+    nlh::launch_build(b, h->stream);
+    nlh::launch_count(b, want_R, h->wide, h->stream);
+    nlh::launch_scan_pairs(b, h->stream);
+    nlh::launch_fill(b, want_R, h->wide, h->stream);
+    hipGraph_t g = nullptr;
+    const hipError_t le = hipGetLastError();
+    const hipError_t ee = hipStreamEndCapture(h->stream, &g);
+    HIPC(le);
+    HIPC(ee);
+    h->graph = g;
+    HIPC(hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0));
+    h->graph_bufs = b;
+    h->graph_want_R = want_R;
+}
+
+// Rebuild through the captured graph.  Returns false when the pair outputs
+// or slots were too small (the fill skipped itself); the caller then runs
+// the eager pipeline, which grows them.
+bool rebuild_graph(nlh_handle *h, bool want_R)
+{
+    // size every pair-stage buffer for this request (R may be asked for now
+    // and not at the build that sized them)
+    ensure_pair_buffers(h);
+    ensure_pair_outputs(h, std::max(h->pair_capacity, estimate_pairs(h)), want_R);
+    ensure_slots(h, std::max(h->slot_cap, estimate_slot_cap(h)), want_R);
+    check_pair_buffers(h, want_R);
+    const nlh::PipelineBufs b = h->bufs();
+    if (!h->graph_exec || h->graph_want_R != want_R || std::memcmp(&b, &h->graph_bufs, sizeof b) != 0)
+        capture_graph(h, want_R);
+    HIPC(hipGraphLaunch(h->graph_exec, h->stream));
+    ++h->graph_launches;
+    h->built = true;
+    h->materialized = false;
+    h->ev_build = h->ev_pairs = false;
+    finish_status(h);
+    check_positions(h);
+    const nlh::Status &st = *h->status_host;
+    if (st.npairs > h->pair_capacity || st.max_count > h->slot_cap || (st.wide && !h->wide)) return false;
+    finish_pairs(h, want_R);
+    return true;
+}
+
+// Full rebuild of the current batch on the eager path.
+void rebuild_eager(nlh_handle *h, uint32_t flags)
+{
+    reset_status(h);
+    build_cells(h);
+    materialize(h, flags);
+}
+
+void validate_tuning(const nlh_tuning &t)
+{
+    if (t.brick_atoms < 64 || t.brick_atoms > 2048)
+        throw Error(NLH_EINVAL, "brick_atoms must be in [64, 2048] (the LDS holds 2048 halo atoms)");
+    if (t.fill_block < 64 || t.fill_block > 1024 || t.fill_block % 64 != 0)
+        throw Error(NLH_EINVAL, "fill_block must be a multiple of 64 in [64, 1024]");
+}
+
+void apply_tuning(nlh_handle *h, const nlh_tuning &t)
+{
+    validate_tuning(t);
+    const bool bricks = t.brick_atoms != h->tune.brick_atoms;
+    h->tune = t;
+    if (bricks && !h->sys_host.empty()) {
+        assign_bricks(h);
+        upload_tables(h);
+    }
+    drop_graph(h);
 }
 
 }  // namespace
@@ -994,10 +1126,78 @@ int nlh_batch_rebuild(nlh_handle *h, const double *X_batch, uint32_t flags, int6
         if (h->sys_host.empty()) throw Error(NLH_ESTATE, "no previous batch to rebuild");
         HIPC(hipSetDevice(h->device));
         upload_positions(h, X_batch, flags);
-        reset_status(h);
-        build_cells(h);
-        materialize(h, flags);
+        const bool want_R = (flags & NLH_WANT_R) != 0;
+        const bool graph = !(flags & NLH_NO_GRAPH) && !h->profile && h->materialized && h->nat > 0;
+        if (!(graph && rebuild_graph(h, want_R))) rebuild_eager(h, flags);
         if (total_pairs) *total_pairs = h->npairs;
+    });
+}
+
+int nlh_set_tuning(nlh_handle *h, const nlh_tuning *t)
+{
+    return guarded([&] {
+        if (!h || !t) throw Error(NLH_EINVAL, "NULL argument");
+        HIPC(hipSetDevice(h->device));
+        apply_tuning(h, *t);
+    });
+}
+
+int nlh_get_tuning(nlh_handle *h, nlh_tuning *t)
+{
+    return guarded([&] {
+        if (!h || !t) throw Error(NLH_EINVAL, "NULL argument");
+        *t = h->tune;
+    });
+}
+
+int nlh_autotune(nlh_handle *h, const double *X_batch, uint32_t flags, int32_t reps, nlh_tuning *best)
+{
+    return guarded([&] {
+        if (!h) throw Error(NLH_EINVAL, "handle is NULL");
+        if (h->sys_host.empty()) throw Error(NLH_ESTATE, "autotune needs a previous build on this handle");
+        if (reps < 1 || reps > 100) throw Error(NLH_EINVAL, "reps must be in [1, 100]");
+        HIPC(hipSetDevice(h->device));
+        upload_positions(h, X_batch, flags);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIPC(hipEventCreate(&e0));
+        HIPC(hipEventCreate(&e1));
+        const nlh_tuning start = h->tune;
+        nlh_tuning win = start;
+        float win_ms = 0.0f;
+        bool have = false;
+        try {
+            static const int32_t bricks[] = {512, 1024, 1536};
+            static const int32_t fills[] = {128, 256, 512};
+            for (int32_t ba : bricks)
+                for (int32_t fb : fills) {
+                    apply_tuning(h, nlh_tuning{ba, fb});
+                    rebuild_eager(h, flags & ~NLH_NO_SYNC);   // warm-up (and buffer growth)
+                    float tot = 0.0f;
+                    for (int r = 0; r < reps; ++r) {
+                        HIPC(hipEventRecord(e0, h->stream));
+                        rebuild_eager(h, flags & ~NLH_NO_SYNC);
+                        HIPC(hipEventRecord(e1, h->stream));
+                        HIPC(hipEventSynchronize(e1));
+                        float ms = 0.0f;
+                        HIPC(hipEventElapsedTime(&ms, e0, e1));
+                        tot += ms;
+                    }
+                    if (!have || tot < win_ms) {
+                        win = h->tune;
+                        win_ms = tot;
+                        have = true;
+                    }
+                }
+            apply_tuning(h, win);
+            rebuild_eager(h, flags & ~NLH_NO_SYNC);   // results of the winning configuration
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (best) *best = win;
     });
 }
 

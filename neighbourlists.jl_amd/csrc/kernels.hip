@@ -257,9 +257,22 @@ __global__ __launch_bounds__(256) void sorted_positions_kernel(const Rec *__rest
     Xs[3 * p + 2] = r.z;
 }
 
+// Zero the build status.  A kernel rather than a memset, so the pipeline can
+// be captured into a HIP graph and replayed.
+__global__ void clear_status_kernel(Status *__restrict__ status)
+{
+    const int t = threadIdx.x;
+    if (t < (int)(sizeof(Status) / 4)) reinterpret_cast<int32_t *>(status)[t] = 0;
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+
+void launch_clear_status(Status *status, hipStream_t st)
+{
+    hipLaunchKernelGGL(clear_status_kernel, dim3(1), dim3(64), 0, st, status);
+}
 
 static inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 

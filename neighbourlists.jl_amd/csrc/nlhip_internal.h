@@ -179,8 +179,10 @@ struct PipelineBufs {
     int64_t pair_capacity;
     Status *status;
     int ti_bytes;
+    int fill_block;          // threads per fill workgroup (multiple of 64)
 };
 
+void launch_clear_status(Status *status, hipStream_t st);
 void launch_build(const PipelineBufs &b, hipStream_t st);
 void launch_count(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st);
 void launch_scan_pairs(const PipelineBufs &b, hipStream_t st);

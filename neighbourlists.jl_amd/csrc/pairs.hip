@@ -623,7 +623,7 @@ __global__ __launch_bounds__(kBrickThreads) void count_kernel(
 // one contiguous range of the output arrays (fill_pairs_kernel!,
 // gpu_kernels.jl:159-180).
 template <bool WANT_R, bool WIDE, typename TI>
-__global__ __launch_bounds__(256) void fill_kernel(
+__global__ __launch_bounds__(1024) void fill_kernel(
     const int64_t *__restrict__ sys_atom_base, int nsys, int64_t nat, const int32_t *__restrict__ pair_count,
     const int64_t *__restrict__ pair_off, const int32_t *__restrict__ inv_perm, const void *__restrict__ slots,
     const double *__restrict__ slotsR, const Status *__restrict__ status, int32_t cap, int64_t pair_capacity,
@@ -704,8 +704,9 @@ void launch_count(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st)
 template <bool R, bool W, typename TI>
 static void fill_launch(const PipelineBufs &b, hipStream_t st)
 {
-    const unsigned nblk = (unsigned)((b.nat + 255) / 256);
-    hipLaunchKernelGGL((fill_kernel<R, W, TI>), dim3(nblk), dim3(256), 0, st, b.sys_atom_base, b.nsys, b.nat,
+    const int bs = b.fill_block;
+    const unsigned nblk = (unsigned)((b.nat + bs - 1) / bs);
+    hipLaunchKernelGGL((fill_kernel<R, W, TI>), dim3(nblk), dim3(bs), 0, st, b.sys_atom_base, b.nsys, b.nat,
                        b.pair_count, b.pair_off, b.inv_perm, b.slots, b.slotsR, b.status, b.slot_cap, b.pair_capacity,
                        (TI *)b.i_out, (TI *)b.j_out, (TI *)b.S_out, b.R_out);
 }

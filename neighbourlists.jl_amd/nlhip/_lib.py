@@ -16,7 +16,7 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "libnlhip.so")
 
 NLH_OK, NLH_EINVAL, NLH_EOVERFLOW, NLH_ENOMEM, NLH_EDEVICE, NLH_ESTATE = range(6)
-NLH_X_DEVICE, NLH_WANT_R, NLH_NO_SYNC = 0x1, 0x2, 0x4
+NLH_X_DEVICE, NLH_WANT_R, NLH_NO_SYNC, NLH_NO_GRAPH = 0x1, 0x2, 0x4, 0x8
 
 
 class NLHError(RuntimeError):
@@ -66,6 +66,14 @@ class Geometry(C.Structure):
         return np.array(self.inv_cell[:]).reshape(3, 3).T.copy()
 
 
+class Tuning(C.Structure):
+    """nlh_tuning: launch configuration (identical results for every value)."""
+    _fields_ = [("brick_atoms", C.c_int32), ("fill_block", C.c_int32)]
+
+    def __repr__(self):
+        return f"Tuning(brick_atoms={self.brick_atoms}, fill_block={self.fill_block})"
+
+
 class PairListView(C.Structure):
     _fields_ = [
         ("nat", C.c_int64),
@@ -99,6 +107,7 @@ EXPORTS = [
     "nlh_geometry_init", "nlh_build_cell_list", "nlh_materialize", "nlh_neighbour_list",
     "nlh_pairlist_device", "nlh_celllist_device", "nlh_copy_pairlist", "nlh_copy_celllist",
     "nlh_neighbours", "nlh_batch_neighbour_list", "nlh_batch_pair_offsets", "nlh_batch_rebuild",
+    "nlh_set_tuning", "nlh_get_tuning", "nlh_autotune",
     "nlh_set_profiling", "nlh_phase_times", "nlh_device_malloc", "nlh_device_free", "nlh_memcpy", "nlh_device_synchronize",
     "nlh_version",
 ]
@@ -125,6 +134,9 @@ def _declare(L):
     L.nlh_batch_neighbour_list.argtypes = [hp, C.c_int32, vp, i64p, C.POINTER(Geometry), C.c_uint32, i64p]
     L.nlh_batch_pair_offsets.argtypes = [hp, i64p]
     L.nlh_batch_rebuild.argtypes = [hp, vp, C.c_uint32, i64p]
+    L.nlh_set_tuning.argtypes = [hp, C.POINTER(Tuning)]
+    L.nlh_get_tuning.argtypes = [hp, C.POINTER(Tuning)]
+    L.nlh_autotune.argtypes = [hp, vp, C.c_uint32, C.c_int32, C.POINTER(Tuning)]
     L.nlh_set_profiling.argtypes = [hp, C.c_int]
     L.nlh_phase_times.argtypes = [hp, C.POINTER(C.c_float)]
     L.nlh_device_malloc.argtypes = [C.c_uint64, C.POINTER(C.c_void_p)]
@@ -256,6 +268,15 @@ class Handle:
 
     def synchronize(self):
         check(lib().nlh_synchronize(self._h), "synchronize")
+
+    def set_tuning(self, brick_atoms: int, fill_block: int):
+        t = Tuning(int(brick_atoms), int(fill_block))
+        check(lib().nlh_set_tuning(self._h, C.byref(t)), "set_tuning")
+
+    def get_tuning(self) -> "Tuning":
+        t = Tuning()
+        check(lib().nlh_get_tuning(self._h, C.byref(t)), "get_tuning")
+        return t
 
     def set_profiling(self, enable: bool = True):
         check(lib().nlh_set_profiling(self._h, 1 if enable else 0), "set_profiling")

@@ -11,7 +11,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import NLH_WANT_R, NLH_X_DEVICE, Geometry, Handle, check, geometry, lib, ti_bytes_of, ti_dtype
+from ._lib import (NLH_NO_GRAPH, NLH_WANT_R, NLH_X_DEVICE, Geometry, Handle, Tuning, check, geometry, lib,
+                   ti_bytes_of, ti_dtype)
 from .api import Int32, PairList
 
 
@@ -101,9 +102,13 @@ class BatchBuilder:
             return int(n.value)
         return self.collect(X, want_R)
 
-    def rebuild(self, X, *, want_R: bool = False, X_device=None, copy: bool = True):
-        """New positions for the same systems (same atom counts and cells)."""
+    def rebuild(self, X, *, want_R: bool = False, X_device=None, copy: bool = True, graph: bool = True):
+        """New positions for the same systems (same atom counts and cells).
+        The pipeline is captured once as a HIP graph and replayed
+        (graph=False runs the stages one by one)."""
         flags = NLH_WANT_R if want_R else 0
+        if not graph:
+            flags |= NLH_NO_GRAPH
         X = np.ascontiguousarray(np.asarray(X, dtype=np.float64))
         ptr = X.ctypes.data
         if X_device is not None:
@@ -114,6 +119,18 @@ class BatchBuilder:
         if not copy:
             return int(n.value)
         return self.collect(X, want_R)
+
+    def autotune(self, X, *, X_device=None, reps: int = 2) -> Tuning:
+        """Pick the fastest launch configuration for the current batch
+        (nlh_autotune); results are identical under every configuration."""
+        X = np.ascontiguousarray(np.asarray(X, dtype=np.float64))
+        ptr, flags = X.ctypes.data, 0
+        if X_device is not None:
+            flags |= NLH_X_DEVICE
+            ptr = _device_ptr(X_device)
+        best = Tuning()
+        check(lib().nlh_autotune(self.handle.ptr, ptr, flags, int(reps), C.byref(best)), "autotune")
+        return best
 
     def collect(self, X, want_R: bool) -> BatchResult:
         nsys = len(self.geoms)

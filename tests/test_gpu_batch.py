@@ -1,0 +1,127 @@
+"""Batched builds: graph-captured rebuilds and launch configurations.
+
+The reference has no batching API, so every system of a batch is checked
+against the single-system oracle (bit for bit: i, j, S, first, R).  A
+graph replay is checked on every replay against the stage-by-stage
+(eager) rebuild of the same positions, with eager work and synchronisations
+between the replays.
+"""
+import numpy as np
+import pytest
+
+import nlhip
+import oracle
+from nlhip import _lib, workloads
+from nlhip.batch import BatchBuilder, pack_systems
+
+pytestmark = pytest.mark.gpu
+
+
+def _same(a, b):
+    assert np.array_equal(a.i, b.i) and np.array_equal(a.j, b.j) and np.array_equal(a.S, b.S)
+    assert np.array_equal(a.first, b.first)
+    if a.R is not None and b.R is not None:
+        assert np.array_equal(np.asarray(a.R).view(np.int64), np.asarray(b.R).view(np.int64))
+
+
+def _jitter(X, seed, sigma):
+    return X + sigma * workloads.normal(seed, X.size).reshape(X.shape)
+
+
+def _check_oracle(res, systems, Xcat, offs, which):
+    for s in which:
+        sysm = systems[s]
+        Xs = Xcat[offs[s]:offs[s + 1]]
+        ref = oracle.neighbour_list(Xs, sysm.cutoff, sysm.cell, sysm.pbc, want_R=True)
+        pl = res.pairlist(s)
+        assert pl.npairs() == ref.npairs
+        assert np.array_equal(pl.i, ref.i) and np.array_equal(pl.j, ref.j) and np.array_equal(pl.S, ref.S)
+        assert np.array_equal(pl.first, ref.first)
+        assert np.array_equal(np.asarray(pl.R).view(np.int64), ref.R.view(np.int64))
+
+
+def test_graph_replays_match_eager():
+    systems = workloads.batch_systems(96, seed=5, nmax=600)
+    X0, offs = pack_systems(systems)
+    bb = BatchBuilder()
+    bb.run(systems, want_R=True)
+    ref_bb = BatchBuilder()
+    ref_bb.run(systems, want_R=True)
+    for rep in range(4):
+        X = _jitter(X0, 100 + rep, 0.1)
+        print(f"graph replay {rep}", flush=True)
+        g = bb.rebuild(X, want_R=True, graph=True)
+        e = ref_bb.rebuild(X, want_R=True, graph=False)          # eager work between replays
+        assert g.npairs == e.npairs
+        _same(g, e)
+        assert np.array_equal(g.pair_offsets, e.pair_offsets)
+        _check_oracle(g, systems, X, offs, [0, 7, 31, 95])
+
+
+def test_graph_replay_with_device_positions():
+    # built without R, rebuilt with R: the replay must size the R slots and
+    # outputs before capturing (regression: they were left unallocated)
+    systems = workloads.batch_systems(32, seed=6, nmax=500)
+    X0, offs = pack_systems(systems)
+    Xd = _lib.DeviceArray(X0)
+    bb = BatchBuilder()
+    bb.run(systems, X_device=Xd, copy=False)
+    for rep in range(3):
+        X = _jitter(X0, 200 + rep, 0.05)
+        Xd.upload(X)
+        g = bb.rebuild(X, X_device=Xd, want_R=True)
+        _check_oracle(g, systems, X, offs, range(32))
+    Xd.free()
+
+
+def test_graph_replay_outgrows_capacity():
+    # compress every system: more pairs than the captured buffers hold, so
+    # the replay's fill skips itself and the stage path regrows the outputs
+    systems = workloads.batch_systems(24, seed=8, nmax=300)
+    X0, offs = pack_systems(systems)
+    bb = BatchBuilder()
+    bb.run(systems)
+    bb.rebuild(X0)                                          # captured here
+    dense = []
+    for s in systems:
+        c = s.X.mean(0)
+        dense.append(workloads.System(c + 0.4 * (s.X - c), s.cell, s.pbc, s.cutoff))
+    Xd_, _ = pack_systems(dense)
+    g = bb.rebuild(Xd_, want_R=True)
+    _check_oracle(g, dense, Xd_, offs, range(24))
+    g2 = bb.rebuild(Xd_, want_R=True)                       # re-captured on the grown buffers
+    _same(g, g2)
+
+
+@pytest.mark.parametrize("brick_atoms,fill_block", [(64, 64), (512, 128), (1024, 512), (2048, 1024)])
+def test_tuning_does_not_change_results(brick_atoms, fill_block):
+    s = workloads.random_cubic(6000, density=0.05, seed=81)
+    h = _lib.Handle()
+    h.set_tuning(brick_atoms, fill_block)
+    t = h.get_tuning()
+    assert (t.brick_atoms, t.fill_block) == (brick_atoms, fill_block)
+    nl = nlhip.neighbour_list(s.X, 5.0, s.cell, (True, False, True), want_R=True, handle=h)
+    ref = oracle.neighbour_list(s.X, 5.0, s.cell, (True, False, True), want_R=True)
+    _same(nl, ref)
+
+
+def test_tuning_rejects_bad_values():
+    h = _lib.Handle()
+    with pytest.raises(_lib.NLHInvalidError):
+        h.set_tuning(16, 256)
+    with pytest.raises(_lib.NLHInvalidError):
+        h.set_tuning(1024, 100)
+
+
+def test_autotune_keeps_results():
+    systems = workloads.batch_systems(128, seed=9, nmax=800)
+    X0, offs = pack_systems(systems)
+    bb = BatchBuilder()
+    before = bb.run(systems, want_R=True)
+    best = bb.autotune(X0, reps=1)
+    assert best.brick_atoms in (512, 1024, 1536) and best.fill_block in (128, 256, 512)
+    t = bb.handle.get_tuning()
+    assert (t.brick_atoms, t.fill_block) == (best.brick_atoms, best.fill_block)
+    after = bb.rebuild(X0, want_R=True)
+    _same(before, after)
+    _check_oracle(after, systems, X0, offs, [0, 1, 64, 127])
