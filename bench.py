@@ -145,6 +145,45 @@ def _batched(nsys: int, steps: int):
             "plain_systems_per_s": round(nsys / tp, 1), "speedup": round(tp / tb, 2)}
 
 
+def _served(nreq: int, seed: int = 11) -> dict:
+    """Request stream (Poisson arrivals; 16..200,000 atoms; 30% rebuilds of a
+    resident system; 64 GB budget) through the continuous-batching server and
+    through the plain per-request call.  Open loop at 75% of the plain path's
+    measured capacity for the latencies, then saturation (all requests
+    available at once) for the capacities."""
+    from nlhip import serve, workloads
+
+    reqs = workloads.served_stream(nreq, rate=1.0, seed=seed)
+    for r in reqs:
+        r.arrival = 0.0
+    warm = reqs[: min(40, nreq)]
+    serve.serve_plain(warm, realtime=False)
+    plain_sat = serve.serve_plain(reqs, realtime=False)
+    sp = serve.summarize(plain_sat)
+    rate = 0.75 * sp["systems_per_s"]
+    gaps = -np.log(1.0 - workloads.uniform(seed + 5, nreq)) / rate
+    t = np.cumsum(gaps) - gaps[0]
+    for r, a in zip(reqs, t):
+        r.arrival = float(a)
+    srv = serve.Server()
+    srv.serve(warm, realtime=False)
+    served_open = srv.serve(reqs, realtime=True)
+    srv.close()
+    plain_open = serve.serve_plain(reqs, realtime=True)
+    for r in reqs:
+        r.arrival = 0.0
+    srv = serve.Server()
+    served_sat = srv.serve(reqs, realtime=False)
+    srv.close()
+    so, po, ss = serve.summarize(served_open), serve.summarize(plain_open), serve.summarize(served_sat)
+    return {"requests": nreq, "mean_atoms": round(float(np.mean([r.nat for r in reqs])), 1),
+            "rebuild_frac": round(float(np.mean([r.rebuild for r in reqs])), 3),
+            "arrival_rate_per_s": round(rate, 1),
+            "open_loop": {"served": so, "plain": po},
+            "saturation": {"served": ss, "plain": sp,
+                           "speedup_systems_per_s": round(ss["systems_per_s"] / sp["systems_per_s"], 2)}}
+
+
 def _cpu_baseline(s) -> dict:
     import oracle
 
@@ -189,6 +228,8 @@ def _only(name: str, steps: int, warmup: int) -> dict:
         return _single(workloads.triclinic_fcc_slab(), steps, warmup)
     if name == "batched":
         return _batched(4096, steps)
+    if name == "served":
+        return _served(steps)
     if name == "copy":
         return _copy_calibration(512 << 20, steps)
     raise SystemExit(f"unknown workload {name}")
@@ -200,7 +241,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--quick", action="store_true", help="headline only")
-    ap.add_argument("--only", choices=["cube_100k", "cube_1M", "fcc_slab_1M", "batched", "copy"],
+    ap.add_argument("--only", choices=["cube_100k", "cube_1M", "fcc_slab_1M", "batched", "served", "copy"],
                     help="run one workload alone (profiler runs); prints its own JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -252,6 +293,7 @@ def main():
                             "phases_ms": {k: round(v, 4) for k, v in sl["phases_ms"].items()}},
         }
         out["batched"] = _batched(4096, 5)
+        out["served"] = _served(600)
         out["cpu_baseline"] = _cpu_baseline(c1)
     print(json.dumps(out), flush=True)
 

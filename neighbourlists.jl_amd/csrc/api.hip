@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <shared_mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -34,6 +35,23 @@ void hip_check(hipError_t e, const char *what)
 }
 #define HIPC(x) hip_check((x), #x)
 
+// Device-wide calls (allocation, free, device or null-stream synchronisation,
+// stream and event creation) are not allowed while any stream of the process
+// is being captured into a graph.  They take this lock shared; a capture
+// (a handful of kernel launches) takes it exclusively.
+std::shared_mutex g_capture_mutex;
+#define HIPD(x)                                                        \
+    do {                                                               \
+        std::shared_lock<std::shared_mutex> lk_(g_capture_mutex);      \
+        hip_check((x), #x);                                            \
+    } while (0)
+template <typename F>
+void device_wide(F &&f)
+{
+    std::shared_lock<std::shared_mutex> lk(g_capture_mutex);
+    f();
+}
+
 template <typename F>
 int guarded(F &&f)
 {
@@ -59,26 +77,27 @@ struct DevBuf {
     template <typename T>
     T *as() const { return static_cast<T *>(p); }
     // returns true when (re)allocated
-    bool ensure(size_t bytes, bool zero = false)
+    // `st` is the owning handle's stream, the only stream that uses the buffer
+    bool ensure(hipStream_t st, size_t bytes, bool zero = false)
     {
         if (bytes == 0) bytes = 16;
         if (bytes <= cap) return false;
-        // growth is rare: wait for every stream before the old block goes
-        HIPC(hipDeviceSynchronize());
-        if (p) HIPC(hipFree(p));
+        // growth is rare: the old block goes once the owner's work on it is done
+        HIPC(hipStreamSynchronize(st));
+        if (p) HIPD(hipFree(p));
         p = nullptr;
         const size_t nb = std::max(bytes + bytes / 8, (size_t)256);
-        HIPC(hipMalloc(&p, nb));
+        HIPD(hipMalloc(&p, nb));
         cap = nb;
         if (zero) {
-            HIPC(hipMemset(p, 0, nb));
-            HIPC(hipDeviceSynchronize());
+            HIPC(hipMemsetAsync(p, 0, nb, st));
+            HIPC(hipStreamSynchronize(st));
         }
         return true;
     }
     void release()
     {
-        if (p) (void)hipFree(p);
+        if (p) device_wide([&] { (void)hipFree(p); });
         p = nullptr;
         cap = 0;
     }
@@ -317,17 +336,19 @@ void ensure_staging(nlh_handle *h, size_t bytes)
 {
     if (bytes <= h->staging_cap) return;
     HIPC(hipStreamSynchronize(h->stream));
-    if (h->staging) HIPC(hipHostFree(h->staging));
+    if (h->staging) HIPD(hipHostFree(h->staging));
     h->staging = nullptr;
     const size_t nb = std::max(bytes * 2, (size_t)4096);
-    HIPC(hipHostMalloc(&h->staging, nb, hipHostMallocDefault));
+    HIPD(hipHostMalloc(&h->staging, nb, hipHostMallocDefault));
     h->staging_cap = nb;
 }
 
 void drop_graph(nlh_handle *h)
 {
-    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
-    if (h->graph) (void)hipGraphDestroy(h->graph);
+    device_wide([&] {
+        if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+        if (h->graph) (void)hipGraphDestroy(h->graph);
+    });
     h->graph_exec = nullptr;
     h->graph = nullptr;
 }
@@ -434,23 +455,23 @@ void setup_batch(nlh_handle *h, int32_t nsys, const int64_t *atom_offsets, const
 
     const size_t n1 = (size_t)std::max<int64_t>(nat, 1);
     const size_t M1 = (size_t)cells + 1;
-    h->sys.ensure(nsys * sizeof(nlh::SysDev));
-    h->atom_base.ensure((nsys + 1) * sizeof(int64_t));
-    h->cell_base.ensure((nsys + 1) * sizeof(int32_t));
-    h->brick_base.ensure((nsys + 1) * sizeof(int32_t));
-    h->key.ensure(n1 * 4);
-    h->slot.ensure(n1 * 4);
-    h->cell_count.ensure(M1 * 4, /*zero=*/true);
-    h->cell_start.ensure(M1 * 4);
-    h->tmp.ensure(n1 * 4);
-    h->inv_perm.ensure(n1 * 4);
-    h->rec.ensure(n1 * sizeof(nlh::Rec));
-    h->wbig.ensure(n1 * 12);
-    h->perm.ensure(n1 * ti);
-    h->cell_id.ensure(n1 * ti);
-    h->cell_offsets.ensure((M1 + nsys) * ti);
-    h->partials.ensure((std::max(M1, n1) / nlh::kScanTile + 2) * sizeof(int64_t));
-    h->status.ensure(sizeof(nlh::Status));
+    h->sys.ensure(h->stream, nsys * sizeof(nlh::SysDev));
+    h->atom_base.ensure(h->stream, (nsys + 1) * sizeof(int64_t));
+    h->cell_base.ensure(h->stream, (nsys + 1) * sizeof(int32_t));
+    h->brick_base.ensure(h->stream, (nsys + 1) * sizeof(int32_t));
+    h->key.ensure(h->stream, n1 * 4);
+    h->slot.ensure(h->stream, n1 * 4);
+    h->cell_count.ensure(h->stream, M1 * 4, /*zero=*/true);
+    h->cell_start.ensure(h->stream, M1 * 4);
+    h->tmp.ensure(h->stream, n1 * 4);
+    h->inv_perm.ensure(h->stream, n1 * 4);
+    h->rec.ensure(h->stream, n1 * sizeof(nlh::Rec));
+    h->wbig.ensure(h->stream, n1 * 12);
+    h->perm.ensure(h->stream, n1 * ti);
+    h->cell_id.ensure(h->stream, n1 * ti);
+    h->cell_offsets.ensure(h->stream, (M1 + nsys) * ti);
+    h->partials.ensure(h->stream, (std::max(M1, n1) / nlh::kScanTile + 2) * sizeof(int64_t));
+    h->status.ensure(h->stream, sizeof(nlh::Status));
 
     upload_tables(h);
 }
@@ -461,7 +482,7 @@ void upload_positions(nlh_handle *h, const double *X, uint32_t flags)
     if (flags & NLH_X_DEVICE) {
         h->X_dev = X;
     } else {
-        h->X.ensure((size_t)std::max<int64_t>(h->nat, 1) * 24);
+        h->X.ensure(h->stream, (size_t)std::max<int64_t>(h->nat, 1) * 24);
         if (h->nat > 0) HIPC(hipMemcpyAsync(h->X.p, X, (size_t)h->nat * 24, hipMemcpyHostToDevice, h->stream));
         h->X_dev = h->X.as<double>();
     }
@@ -525,17 +546,17 @@ void ensure_slots(nlh_handle *h, int32_t cap, bool want_R)
 {
     h->slot_cap = std::max(h->slot_cap, cap);
     const size_t n1 = (size_t)std::max<int64_t>(h->nat, 1) * (size_t)h->slot_cap;
-    h->slots.ensure(n1 * (h->wide ? 16 : 8));
-    if (want_R) h->slotsR.ensure(n1 * 24);
+    h->slots.ensure(h->stream, n1 * (h->wide ? 16 : 8));
+    if (want_R) h->slotsR.ensure(h->stream, n1 * 24);
 }
 
 void ensure_pair_outputs(nlh_handle *h, int64_t cap, bool want_R)
 {
     const int ti = h->ti_bytes;
-    h->i.ensure((size_t)cap * ti);
-    h->j.ensure((size_t)cap * ti);
-    h->S.ensure((size_t)cap * 3 * ti);
-    if (want_R) h->R.ensure((size_t)cap * 24);
+    h->i.ensure(h->stream, (size_t)cap * ti);
+    h->j.ensure(h->stream, (size_t)cap * ti);
+    h->S.ensure(h->stream, (size_t)cap * 3 * ti);
+    if (want_R) h->R.ensure(h->stream, (size_t)cap * 24);
     int64_t c = (int64_t)(h->i.cap / ti);
     c = std::min<int64_t>(c, (int64_t)(h->j.cap / ti));
     c = std::min<int64_t>(c, (int64_t)(h->S.cap / (3 * ti)));
@@ -546,10 +567,10 @@ void ensure_pair_outputs(nlh_handle *h, int64_t cap, bool want_R)
 void ensure_pair_buffers(nlh_handle *h)
 {
     const size_t n1 = (size_t)std::max<int64_t>(h->nat, 1);
-    h->pair_count.ensure(n1 * 4);
-    h->pair_off.ensure((n1 + 1) * 8);
-    h->first.ensure((n1 + h->nsys) * h->ti_bytes);
-    h->sys_pair_base.ensure((h->nsys + 1) * 8);
+    h->pair_count.ensure(h->stream, n1 * 4);
+    h->pair_off.ensure(h->stream, (n1 + 1) * 8);
+    h->first.ensure(h->stream, (n1 + h->nsys) * h->ti_bytes);
+    h->sys_pair_base.ensure(h->stream, (h->nsys + 1) * 8);
 }
 
 void finish_status(nlh_handle *h)
@@ -597,10 +618,10 @@ void materialize(nlh_handle *h, uint32_t flags)
         h->npairs = 0;
         if (h->nsys == 1) {
             const int64_t one = 1;
-            if (h->ti_bytes == 8) HIPC(hipMemcpy(h->first.p, &one, 8, hipMemcpyHostToDevice));
+            if (h->ti_bytes == 8) HIPD(hipMemcpy(h->first.p, &one, 8, hipMemcpyHostToDevice));
             else {
                 const int32_t o = 1;
-                HIPC(hipMemcpy(h->first.p, &o, 4, hipMemcpyHostToDevice));
+                HIPD(hipMemcpy(h->first.p, &o, 4, hipMemcpyHostToDevice));
             }
         }
         h->materialized = true;
@@ -654,7 +675,7 @@ void finish_pairs(nlh_handle *h, bool want_R)
         h->pair_offsets_host[0] = 0;
         h->pair_offsets_host[1] = h->npairs;
     } else {
-        HIPC(hipMemcpy(h->pair_offsets_host.data(), h->sys_pair_base.p, (h->nsys + 1) * 8, hipMemcpyDeviceToHost));
+        HIPD(hipMemcpy(h->pair_offsets_host.data(), h->sys_pair_base.p, (h->nsys + 1) * 8, hipMemcpyDeviceToHost));
     }
     if (h->ti_bytes == 4)
         for (int s = 0; s < h->nsys; ++s)
@@ -672,19 +693,24 @@ void capture_graph(nlh_handle *h, bool want_R)
 {
     drop_graph(h);
     const nlh::PipelineBufs b = h->bufs();
-    HIPC(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    nlh::launch_clear_status(b.status, h->stream);
-    nlh::launch_build(b, h->stream);
-    nlh::launch_count(b, want_R, h->wide, h->stream);
-    nlh::launch_scan_pairs(b, h->stream);
-    nlh::launch_fill(b, want_R, h->wide, h->stream);
     hipGraph_t g = nullptr;
-    const hipError_t le = hipGetLastError();
-    const hipError_t ee = hipStreamEndCapture(h->stream, &g);
+    hipError_t le = hipSuccess, ee = hipSuccess;
+    {
+        // kernel launches only between begin and end (nothing that takes the lock)
+        std::unique_lock<std::shared_mutex> lk(g_capture_mutex);
+        HIPC(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        nlh::launch_clear_status(b.status, h->stream);
+        nlh::launch_build(b, h->stream);
+        nlh::launch_count(b, want_R, h->wide, h->stream);
+        nlh::launch_scan_pairs(b, h->stream);
+        nlh::launch_fill(b, want_R, h->wide, h->stream);
+        le = hipGetLastError();
+        ee = hipStreamEndCapture(h->stream, &g);
+    }
     HIPC(le);
     HIPC(ee);
     h->graph = g;
-    HIPC(hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0));
+    HIPD(hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0));
     h->graph_bufs = b;
     h->graph_want_R = want_R;
 }
@@ -777,14 +803,14 @@ int nlh_device_malloc(uint64_t bytes, void **ptr)
         if (!ptr) throw Error(NLH_EINVAL, "ptr is NULL");
         HIPC(hipSetDevice(g_device));
         *ptr = nullptr;
-        HIPC(hipMalloc(ptr, bytes ? bytes : 16));
+        HIPD(hipMalloc(ptr, bytes ? bytes : 16));
     });
 }
 
 int nlh_device_free(void *ptr)
 {
     return guarded([&] {
-        if (ptr) HIPC(hipFree(ptr));
+        if (ptr) HIPD(hipFree(ptr));
     });
 }
 
@@ -793,7 +819,7 @@ int nlh_memcpy(void *dst, const void *src, uint64_t bytes)
     return guarded([&] {
         if (bytes == 0) return;
         if (!dst || !src) throw Error(NLH_EINVAL, "NULL pointer");
-        HIPC(hipMemcpy(dst, src, bytes, hipMemcpyDefault));
+        HIPD(hipMemcpy(dst, src, bytes, hipMemcpyDefault));
     });
 }
 
@@ -801,7 +827,7 @@ int nlh_device_synchronize(void)
 {
     return guarded([&] {
         HIPC(hipSetDevice(g_device));
-        HIPC(hipDeviceSynchronize());
+        HIPD(hipDeviceSynchronize());
     });
 }
 
@@ -836,8 +862,8 @@ int nlh_create(nlh_handle **out)
         h->device = g_device;
         try {
             HIPC(hipSetDevice(h->device));
-            HIPC(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-            HIPC(hipHostMalloc((void **)&h->status_host, sizeof(nlh::Status), hipHostMallocDefault));
+            HIPD(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+            HIPD(hipHostMalloc((void **)&h->status_host, sizeof(nlh::Status), hipHostMallocDefault));
             std::memset(h->status_host, 0, sizeof(nlh::Status));
         } catch (...) {
             delete h;
@@ -854,17 +880,21 @@ int nlh_destroy(nlh_handle *h)
         (void)hipSetDevice(h->device);
         if (h->stream) (void)hipStreamSynchronize(h->stream);
         drop_graph(h);
-        for (hipEvent_t &e : h->ev)
-            if (e) (void)hipEventDestroy(e);
+        device_wide([&] {
+            for (hipEvent_t &e : h->ev)
+                if (e) (void)hipEventDestroy(e);
+        });
         DevBuf *bufs[] = {&h->X,        &h->sys,       &h->atom_base,  &h->cell_base, &h->brick_base,    &h->key,    &h->slot,
                           &h->cell_count, &h->cell_start, &h->tmp,      &h->rec,          &h->wbig,   &h->perm,
                           &h->cell_id,  &h->cell_offsets, &h->pair_count, &h->inv_perm, &h->slots, &h->slotsR, &h->pair_off,   &h->partials,
                           &h->sys_pair_base, &h->first, &h->i,          &h->j,            &h->S,      &h->R,
                           &h->status,   &h->Xsorted,   &h->nb_j,       &h->nb_R,         &h->nb_S,   &h->nb_count};
         for (DevBuf *b : bufs) b->release();
-        if (h->status_host) (void)hipHostFree(h->status_host);
-        if (h->staging) (void)hipHostFree(h->staging);
-        if (h->stream) (void)hipStreamDestroy(h->stream);
+        device_wide([&] {
+            if (h->status_host) (void)hipHostFree(h->status_host);
+            if (h->staging) (void)hipHostFree(h->staging);
+            if (h->stream) (void)hipStreamDestroy(h->stream);
+        });
         delete h;
     });
 }
@@ -985,15 +1015,15 @@ int nlh_copy_pairlist(nlh_handle *h, void *i, void *j, void *S, void *first, dou
         HIPC(hipStreamSynchronize(h->stream));
         const size_t P = (size_t)h->npairs, ti = (size_t)h->ti_bytes;
         if (P > 0) {
-            if (i) HIPC(hipMemcpy(i, h->i.p, P * ti, hipMemcpyDefault));
-            if (j) HIPC(hipMemcpy(j, h->j.p, P * ti, hipMemcpyDefault));
-            if (S) HIPC(hipMemcpy(S, h->S.p, 3 * P * ti, hipMemcpyDefault));
+            if (i) HIPD(hipMemcpy(i, h->i.p, P * ti, hipMemcpyDefault));
+            if (j) HIPD(hipMemcpy(j, h->j.p, P * ti, hipMemcpyDefault));
+            if (S) HIPD(hipMemcpy(S, h->S.p, 3 * P * ti, hipMemcpyDefault));
             if (R) {
                 if (!h->have_R) throw Error(NLH_ESTATE, "R was not requested (NLH_WANT_R)");
-                HIPC(hipMemcpy(R, h->R.p, 3 * P * 8, hipMemcpyDefault));
+                HIPD(hipMemcpy(R, h->R.p, 3 * P * 8, hipMemcpyDefault));
             }
         }
-        if (first) HIPC(hipMemcpy(first, h->first.p, ((size_t)h->nat + h->nsys) * ti, hipMemcpyDefault));
+        if (first) HIPD(hipMemcpy(first, h->first.p, ((size_t)h->nat + h->nsys) * ti, hipMemcpyDefault));
     });
 }
 
@@ -1005,16 +1035,16 @@ int nlh_copy_celllist(nlh_handle *h, double *X_sorted, void *perm, void *cell_id
         HIPC(hipSetDevice(h->device));
         const size_t n = (size_t)h->nat, ti = (size_t)h->ti_bytes;
         if (X_sorted && n > 0) {
-            h->Xsorted.ensure(n * 24);
+            h->Xsorted.ensure(h->stream, n * 24);
             nlh::launch_sorted_positions(h->bufs(), h->Xsorted.as<double>(), h->stream);
             HIPC(hipGetLastError());
         }
         HIPC(hipStreamSynchronize(h->stream));
-        if (X_sorted && n > 0) HIPC(hipMemcpy(X_sorted, h->Xsorted.p, n * 24, hipMemcpyDefault));
-        if (perm && n > 0) HIPC(hipMemcpy(perm, h->perm.p, n * ti, hipMemcpyDefault));
-        if (cell_id && n > 0) HIPC(hipMemcpy(cell_id, h->cell_id.p, n * ti, hipMemcpyDefault));
+        if (X_sorted && n > 0) HIPD(hipMemcpy(X_sorted, h->Xsorted.p, n * 24, hipMemcpyDefault));
+        if (perm && n > 0) HIPD(hipMemcpy(perm, h->perm.p, n * ti, hipMemcpyDefault));
+        if (cell_id && n > 0) HIPD(hipMemcpy(cell_id, h->cell_id.p, n * ti, hipMemcpyDefault));
         if (cell_offsets)
-            HIPC(hipMemcpy(cell_offsets, h->cell_offsets.p, ((size_t)h->M + h->nsys) * ti, hipMemcpyDefault));
+            HIPD(hipMemcpy(cell_offsets, h->cell_offsets.p, ((size_t)h->M + h->nsys) * ti, hipMemcpyDefault));
     });
 }
 
@@ -1030,10 +1060,10 @@ int nlh_neighbours(nlh_handle *h, int32_t sys, int64_t i, int64_t cap, void *j, 
         HIPC(hipSetDevice(h->device));
         const int ti = h->ti_bytes;
         const size_t c1 = (size_t)std::max<int64_t>(cap, 1);
-        h->nb_j.ensure(c1 * ti);
-        h->nb_R.ensure(c1 * 24);
-        h->nb_S.ensure(c1 * 3 * ti);
-        h->nb_count.ensure(8);
+        h->nb_j.ensure(h->stream, c1 * ti);
+        h->nb_R.ensure(h->stream, c1 * 24);
+        h->nb_S.ensure(h->stream, c1 * 3 * ti);
+        h->nb_count.ensure(h->stream, 8);
         nlh_celllist_view v{};
         v.nat = sd.nat;
         v.ncells_total = sd.M;
@@ -1055,12 +1085,12 @@ int nlh_neighbours(nlh_handle *h, int32_t sys, int64_t i, int64_t cap, void *j, 
         HIPC(hipGetLastError());
         HIPC(hipStreamSynchronize(h->stream));
         int64_t n = 0;
-        HIPC(hipMemcpy(&n, h->nb_count.p, 8, hipMemcpyDeviceToHost));
+        HIPD(hipMemcpy(&n, h->nb_count.p, 8, hipMemcpyDeviceToHost));
         const size_t m = (size_t)std::min(n, cap);
         if (m > 0) {
-            if (j) HIPC(hipMemcpy(j, h->nb_j.p, m * ti, hipMemcpyDeviceToHost));
-            if (R) HIPC(hipMemcpy(R, h->nb_R.p, m * 24, hipMemcpyDeviceToHost));
-            if (S) HIPC(hipMemcpy(S, h->nb_S.p, m * 3 * ti, hipMemcpyDeviceToHost));
+            if (j) HIPD(hipMemcpy(j, h->nb_j.p, m * ti, hipMemcpyDeviceToHost));
+            if (R) HIPD(hipMemcpy(R, h->nb_R.p, m * 24, hipMemcpyDeviceToHost));
+            if (S) HIPD(hipMemcpy(S, h->nb_S.p, m * 3 * ti, hipMemcpyDeviceToHost));
         }
         if (count) *count = n;
     });
@@ -1072,7 +1102,7 @@ int nlh_set_profiling(nlh_handle *h, int enable)
         if (!h) throw Error(NLH_EINVAL, "handle is NULL");
         HIPC(hipSetDevice(h->device));
         if (enable && !h->ev[0])
-            for (hipEvent_t &e : h->ev) HIPC(hipEventCreate(&e));
+            for (hipEvent_t &e : h->ev) HIPD(hipEventCreate(&e));
         h->profile = enable != 0;
     });
 }
@@ -1159,8 +1189,8 @@ int nlh_autotune(nlh_handle *h, const double *X_batch, uint32_t flags, int32_t r
         HIPC(hipSetDevice(h->device));
         upload_positions(h, X_batch, flags);
         hipEvent_t e0 = nullptr, e1 = nullptr;
-        HIPC(hipEventCreate(&e0));
-        HIPC(hipEventCreate(&e1));
+        HIPD(hipEventCreate(&e0));
+        HIPD(hipEventCreate(&e1));
         const nlh_tuning start = h->tune;
         nlh_tuning win = start;
         float win_ms = 0.0f;
@@ -1191,12 +1221,16 @@ int nlh_autotune(nlh_handle *h, const double *X_batch, uint32_t flags, int32_t r
             apply_tuning(h, win);
             rebuild_eager(h, flags & ~NLH_NO_SYNC);   // results of the winning configuration
         } catch (...) {
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
+            device_wide([&] {
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+            });
             throw;
         }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        device_wide([&] {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        });
         if (best) *best = win;
     });
 }

@@ -1,0 +1,399 @@
+"""Serving loop: neighbour-list requests with continuous batching on one GPU.
+
+The reference serves nothing: each `neighbour_list` call builds one system
+(src/cell_list.jl:897-916).  This loop takes a stream of requests -- new
+systems of any size, or new positions for a system served before -- and
+keeps the device busy:
+
+* admission: a request enters the active set only while the estimated
+  device memory of everything in flight plus the resident systems fits the
+  HBM budget; resident systems are evicted least-recently-used first when
+  a new request would not fit;
+* continuous batching: whenever a worker is free, the scheduler hands it the
+  next unit of work -- one large system, or as many admitted small systems
+  as fit `batch_atoms` in a single batched build -- so requests that arrive
+  while others run join the next batch instead of waiting for a drain;
+* overlap: each worker thread owns its own handle (its own HIP stream and
+  buffers); the C calls release the GIL, so several builds are on the
+  device at once;
+* residency: a large system keeps its positions and its cell list on the
+  device in a handle of its own; a rebuild request uploads the new
+  positions into the resident buffer and replays the captured pipeline
+  (nlh_batch_rebuild).
+
+Every result equals the single-system build of the same input bit for bit
+(the batch and replay paths emit in the reference traversal order), which
+the alone-vs-mixed test checks.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import queue
+import threading
+import time
+from collections import OrderedDict
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import NLH_X_DEVICE, DeviceArray, Handle, check, geometry, lib
+from .api import Int32, PairList
+from .batch import BatchBuilder, pack_systems
+from .workloads import System
+
+
+@dataclass
+class Request:
+    rid: int
+    system_id: int
+    X: np.ndarray
+    cell: np.ndarray
+    pbc: tuple
+    cutoff: float
+    rebuild: bool = False     # new positions for a system served before
+    arrival: float = 0.0      # seconds from the start of the stream
+
+    @property
+    def nat(self) -> int:
+        return int(self.X.shape[0])
+
+
+@dataclass
+class Completion:
+    rid: int
+    system_id: int
+    npairs: int
+    arrival: float
+    start: float
+    end: float
+    batch: int                # systems in the unit of work that served it
+    result: PairList | None = None
+
+    @property
+    def latency(self) -> float:
+        return self.end - self.arrival
+
+
+def estimate_bytes(nat: int, cell, cutoff: float, ti_bytes: int = 4) -> int:
+    """Device memory of one build: positions, cell list, neighbour slots and
+    pair outputs (pair count from the mean density; over-estimates by ~25%)."""
+    cell = np.asarray(cell, dtype=np.float64)
+    vol = abs(float(np.linalg.det(cell)))
+    rho = nat / vol if vol > 0 else 0.0
+    nb = rho * 4.18879020478639 * cutoff ** 3
+    heights = np.array([vol / max(np.linalg.norm(np.cross(cell[(k + 1) % 3], cell[(k + 2) % 3])), 1e-300)
+                        for k in range(3)])
+    M = float(np.prod(np.maximum(np.floor(heights / cutoff), 1.0)))
+    cap = int((nb * 1.5 + 16 + 7) // 8 * 8)
+    pairs = nat * (nb * 1.25 + 4)
+    per_atom = 24 + 6 * 4 + 32 + 12 + 2 * ti_bytes + 8 + ti_bytes + 8 * cap
+    b = nat * per_atom + M * (12 + ti_bytes) + pairs * 5 * ti_bytes
+    return int(b * 1.15) + (1 << 20)
+
+
+class _Resident:
+    """A large system kept on the device: positions, handle (cell list,
+    pair list, captured pipeline)."""
+
+    def __init__(self, req: Request, int_type):
+        self.system_id = req.system_id
+        self.nat = req.nat
+        self.geom = geometry(req.cell, req.cutoff, req.pbc, _lib.ti_bytes_of(int_type))
+        self.cell, self.pbc, self.cutoff = req.cell, req.pbc, req.cutoff
+        self.handle = Handle()
+        self.X = DeviceArray(np.ascontiguousarray(req.X, dtype=np.float64))
+        self.bytes = estimate_bytes(req.nat, req.cell, req.cutoff, _lib.ti_bytes_of(int_type))
+        self.lock = threading.Lock()
+        self.built = False
+
+    def build(self, X: np.ndarray | None) -> int:
+        n = C.c_int64()
+        if X is not None:
+            self.X.upload(X)
+        if self.built:
+            check(lib().nlh_batch_rebuild(self.handle.ptr, self.X.ptr, NLH_X_DEVICE, C.byref(n)), "rebuild")
+        else:
+            check(lib().nlh_neighbour_list(self.handle.ptr, self.X.ptr, self.nat, C.byref(self.geom), NLH_X_DEVICE,
+                                           C.byref(n)), "neighbour_list")
+            self.built = True
+        return int(n.value)
+
+    def result(self, X_host: np.ndarray) -> PairList:
+        from .api import _collect_pairs
+        v = _lib.PairListView()
+        check(lib().nlh_pairlist_device(self.handle.ptr, C.byref(v)))
+        return _collect_pairs(self.handle, self.geom, X_host, int(v.npairs), False, None)
+
+    def free(self):
+        self.X.free()
+        self.handle.close()
+
+
+class Server:
+    """Continuous-batching neighbour-list server on one GPU.
+
+    hbm_budget    bytes of device memory the server may hold (in flight plus
+                  resident systems)
+    workers       worker threads, each with its own handle (HIP stream and
+                  grow-only buffers, reused from one unit of work to the next)
+    small_atoms   systems below this size are batched together; larger ones
+                  are resident (own handle, positions and cell list kept on
+                  the device, rebuilt by graph replay)
+    batch_atoms   atoms per batched build
+    keep_results  copy each result back to the host (PairList) on completion
+    """
+
+    def __init__(self, hbm_budget: int = 64 << 30, workers: int = 3, small_atoms: int = 65536,
+                 batch_atoms: int = 1 << 21, int_type=Int32, keep_results: bool = False):
+        self.budget = int(hbm_budget)
+        self.nworkers = int(workers)
+        self.small_atoms = int(small_atoms)
+        self.batch_atoms = int(batch_atoms)
+        self.int_type = int_type
+        self.keep = keep_results
+        self.residents: OrderedDict[int, _Resident] = OrderedDict()
+        self.resident_bytes = 0
+        self.inflight_bytes = 0
+        self.peak_bytes = 0          # largest in-flight + resident estimate admitted
+        self.evictions = 0
+        self._lock = threading.Lock()
+        self._work: queue.Queue = queue.Queue()
+        self._done: queue.Queue = queue.Queue()
+        self._free_workers = threading.Semaphore(0)
+        self._threads: list[threading.Thread] = []
+        self._errors: list[BaseException] = []
+        self.t0 = 0.0
+
+    # ------------------------------------------------------------------ workers
+
+    def _worker(self):
+        local = {"bb": None}
+        self._free_workers.release()
+        while True:
+            item = self._work.get()
+            if item is None:
+                return
+            try:
+                self._run_item(item, local)
+            except BaseException as e:   # reported by serve()
+                self._errors.append(e)
+                for r in item[1]:
+                    self._done.put(("error", r, 0))
+            finally:
+                self._free_workers.release()
+
+    def _run_item(self, item, local):
+        kind, reqs, nbytes = item
+        start = time.perf_counter() - self.t0
+        if kind == "large":
+            r = reqs[0]
+            res = self._resident_for(r, nbytes)
+            nbytes = 0                       # the reservation became the resident's
+            with res.lock:
+                P = res.build(np.ascontiguousarray(r.X, dtype=np.float64) if (r.rebuild or res.built) else None)
+                out = res.result(r.X) if self.keep else None
+            end = time.perf_counter() - self.t0
+            self._done.put(("ok", Completion(r.rid, r.system_id, P, r.arrival, start, end, 1, out), nbytes))
+            return
+        if local["bb"] is None:
+            local["bb"] = BatchBuilder(self.int_type)
+        bb: BatchBuilder = local["bb"]
+        systems = [System(r.X, r.cell, r.pbc, r.cutoff) for r in reqs]
+        bb.run(systems, copy=False)
+        po = np.zeros(len(reqs) + 1, dtype=np.int64)
+        check(lib().nlh_batch_pair_offsets(bb.handle.ptr, po.ctypes.data_as(C.POINTER(C.c_int64))))
+        res = None
+        if self.keep:
+            X, _ = pack_systems(systems)
+            res = bb.collect(X, False)
+        end = time.perf_counter() - self.t0
+        for s, r in enumerate(reqs):
+            out = res.pairlist(s) if res is not None else None
+            self._done.put(("ok", Completion(r.rid, r.system_id, int(po[s + 1] - po[s]), r.arrival, start, end,
+                                             len(reqs), out), nbytes if s == 0 else 0))
+
+    def _resident_for(self, r: Request, reserved: int) -> _Resident:
+        """The resident entry of r's system; a new one takes over the bytes
+        reserved for it at admission."""
+        with self._lock:
+            res = self.residents.get(r.system_id)
+            if res is not None:
+                self.residents.move_to_end(r.system_id)
+                self.inflight_bytes -= reserved
+                if res.nat != r.nat:
+                    raise ValueError(f"rebuild of system {r.system_id} with a different atom count")
+                return res
+        res = _Resident(r, self.int_type)
+        with self._lock:
+            self.residents[r.system_id] = res
+            self.inflight_bytes -= reserved
+            self.resident_bytes += res.bytes
+        return res
+
+    # ---------------------------------------------------------------- scheduler
+
+    def _fits(self, nbytes: int) -> bool:
+        with self._lock:
+            if self.inflight_bytes + self.resident_bytes + nbytes <= self.budget:
+                return True
+            # evict idle residents, least recently used first
+            for sid in list(self.residents):
+                if self.inflight_bytes + self.resident_bytes + nbytes <= self.budget:
+                    break
+                res = self.residents[sid]
+                if res.lock.acquire(blocking=False):
+                    try:
+                        del self.residents[sid]
+                        self.resident_bytes -= res.bytes
+                        self.evictions += 1
+                        res.free()
+                    finally:
+                        res.lock.release()
+            return self.inflight_bytes + self.resident_bytes + nbytes <= self.budget
+
+    def serve(self, requests: list[Request], *, realtime: bool = True) -> list[Completion]:
+        """Serve a request stream.  realtime=True releases each request at its
+        arrival time (open loop); False makes every request available at once
+        (saturation)."""
+        order = sorted(requests, key=lambda r: r.arrival)
+        for r in order:
+            if estimate_bytes(r.nat, r.cell, r.cutoff) > self.budget:
+                raise _lib.NLHMemoryError(_lib.NLH_ENOMEM, f"request {r.rid} ({r.nat} atoms) alone exceeds the "
+                                                           f"HBM budget of {self.budget} bytes")
+        self._threads = [threading.Thread(target=self._worker, daemon=True) for _ in range(self.nworkers)]
+        for t in self._threads:
+            t.start()
+        out: list[Completion] = []
+        waiting: list[Request] = []
+        k = 0
+        n = len(order)
+        self.t0 = time.perf_counter()
+        inflight_systems = set()
+        try:
+            while len(out) < n:
+                now = time.perf_counter() - self.t0
+                while k < n and (not realtime or order[k].arrival <= now):
+                    waiting.append(order[k])
+                    k += 1
+                # completions release memory
+                while True:
+                    try:
+                        st, c, nb = self._done.get_nowait()
+                    except queue.Empty:
+                        break
+                    if st == "error":
+                        raise self._errors[0]
+                    with self._lock:
+                        self.inflight_bytes -= nb
+                    inflight_systems.discard(c.system_id)
+                    out.append(c)
+                # dispatch while a worker is free
+                dispatched = False
+                while waiting and self._free_workers.acquire(blocking=False):
+                    item = self._next_item(waiting, inflight_systems)
+                    if item is None:
+                        self._free_workers.release()
+                        break
+                    self._work.put(item)
+                    dispatched = True
+                if not dispatched:
+                    if k < n and realtime:
+                        time.sleep(min(max(order[k].arrival - (time.perf_counter() - self.t0), 0.0), 2e-4))
+                    else:
+                        time.sleep(2e-5)
+        finally:
+            for _ in self._threads:
+                self._work.put(None)
+            for t in self._threads:
+                t.join(timeout=60)
+        return sorted(out, key=lambda c: c.rid)
+
+    def _next_item(self, waiting: list[Request], inflight_systems: set):
+        """The next unit of work from the admitted head of the queue: one
+        large system, or a batch of small ones (FIFO order)."""
+        for idx, r in enumerate(waiting):
+            if r.system_id in inflight_systems:
+                continue                     # a system is built by one worker at a time
+            if r.nat >= self.small_atoms:
+                res = self.residents.get(r.system_id)
+                nb = 0 if res is not None else estimate_bytes(r.nat, r.cell, r.cutoff)
+                if not self._fits(nb):
+                    return None
+                with self._lock:
+                    self.inflight_bytes += nb
+                    self.peak_bytes = max(self.peak_bytes, self.inflight_bytes + self.resident_bytes)
+                del waiting[idx]
+                inflight_systems.add(r.system_id)
+                return ("large", [r], nb)
+            batch, atoms, nb = [], 0, 0
+            rest = []
+            for q in waiting[idx:]:
+                if (q.nat < self.small_atoms and q.system_id not in inflight_systems and
+                        atoms + q.nat <= self.batch_atoms and q.system_id not in {b.system_id for b in batch}):
+                    e = estimate_bytes(q.nat, q.cell, q.cutoff)
+                    if not self._fits(nb + e):
+                        rest.append(q)
+                        continue
+                    batch.append(q)
+                    atoms += q.nat
+                    nb += e
+                else:
+                    rest.append(q)
+            if not batch:
+                return None
+            waiting[idx:] = rest
+            with self._lock:
+                self.inflight_bytes += nb
+                self.peak_bytes = max(self.peak_bytes, self.inflight_bytes + self.resident_bytes)
+            for q in batch:
+                inflight_systems.add(q.system_id)
+            return ("batch", batch, nb)
+        return None
+
+    def close(self):
+        with self._lock:
+            for res in self.residents.values():
+                res.free()
+            self.residents.clear()
+            self.resident_bytes = 0
+
+
+def serve_plain(requests: list[Request], *, realtime: bool = True, int_type=Int32,
+                keep_results: bool = False) -> list[Completion]:
+    """The plain per-request path: one single-system build call per request,
+    in arrival order, each waiting for its result (nlh_neighbour_list on
+    host positions)."""
+    h = Handle()
+    ti = _lib.ti_bytes_of(int_type)
+    order = sorted(requests, key=lambda r: r.arrival)
+    out = []
+    t0 = time.perf_counter()
+    n = C.c_int64()
+    for r in order:
+        if realtime:
+            while time.perf_counter() - t0 < r.arrival:
+                time.sleep(min(r.arrival - (time.perf_counter() - t0), 2e-4) if r.arrival > 0 else 0)
+        start = time.perf_counter() - t0
+        g = geometry(r.cell, r.cutoff, r.pbc, ti)
+        X = np.ascontiguousarray(r.X, dtype=np.float64)
+        check(lib().nlh_neighbour_list(h.ptr, X.ctypes.data, X.shape[0], C.byref(g), 0, C.byref(n)), "plain")
+        res = None
+        if keep_results:
+            from .api import _collect_pairs
+            res = _collect_pairs(h, g, X, int(n.value), False, None)
+        end = time.perf_counter() - t0
+        out.append(Completion(r.rid, r.system_id, int(n.value), r.arrival, start, end, 1, res))
+    h.close()
+    return sorted(out, key=lambda c: c.rid)
+
+
+def summarize(done: list[Completion], wall: float | None = None) -> dict:
+    lat = np.array([c.latency for c in done]) * 1e3
+    t_end = max(c.end for c in done)
+    t_first = min(c.arrival for c in done)
+    span = wall if wall is not None else (t_end - t_first)
+    pairs = sum(c.npairs for c in done)
+    return {"requests": len(done), "span_s": round(span, 4), "systems_per_s": round(len(done) / span, 1),
+            "pairs_per_s": round(pairs / span, 1), "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3), "mean_batch": round(float(np.mean([c.batch for c in done])), 2)}

@@ -124,3 +124,41 @@ def batch_systems(nsys: int = 4096, seed: int = 7, nmin: int = 8, nmax: int = 20
         X = uniform(seed * 1000003 + s, 3 * N).reshape(N, 3) @ C
         out.append(System(X, C, pbc, float(rc)))
     return out
+
+
+def served_stream(nreq: int, rate: float, seed: int = 11, nmin: int = 16, nmax: int = 200_000,
+                  rebuild_frac: float = 0.30, sigma: float = 0.1, cutoff: float = 5.0):
+    """A stream of neighbour-list requests: Poisson arrivals at `rate` per
+    second; sizes log-uniform in [nmin, nmax]; density U[0.05, 0.09]; cubic
+    or triclinic cells (off-diagonals U[-0.15, 0.15] times the diagonal);
+    per-axis pbc ~ Bernoulli(1/2); cutoff 5.  A fraction `rebuild_frac` of the
+    requests are new positions of a system requested before (every atom
+    displaced by N(0, sigma)).  Returns serve.Request objects."""
+    from .serve import Request
+
+    u = uniform(seed, 12 * nreq).reshape(nreq, 12)
+    gaps = -np.log(1.0 - uniform(seed + 1, nreq)) / rate
+    t = np.cumsum(gaps) - gaps[0]
+    out, systems = [], []
+    for k in range(nreq):
+        r = u[k]
+        if systems and r[0] < rebuild_frac:
+            sid = int(r[1] * len(systems))
+            base = systems[sid]
+            X = base.X + sigma * normal(seed * 7919 + k, base.X.size).reshape(base.X.shape)
+            out.append(Request(k, sid, X, base.cell, base.pbc, base.cutoff, True, float(t[k])))
+            continue
+        N = int(round(np.exp(np.log(nmin) + r[2] * (np.log(nmax) - np.log(nmin)))))
+        rho = 0.05 + 0.04 * r[3]
+        d = (N / rho) ** (1.0 / 3.0)
+        C = np.diag([d, d, d])
+        if r[4] < 0.5:
+            off = (r[5:11] * 0.30 - 0.15) * d
+            C[0, 1], C[0, 2], C[1, 0], C[1, 2], C[2, 0], C[2, 1] = off
+        pbc = tuple(bool(b) for b in (uniform(seed * 31 + k, 3) < 0.5))
+        X = uniform(seed * 1000003 + k, 3 * N).reshape(N, 3) @ C
+        sysm = System(X, C, pbc, float(cutoff))
+        sid = len(systems)
+        systems.append(sysm)
+        out.append(Request(k, sid, X, C, pbc, float(cutoff), False, float(t[k])))
+    return out

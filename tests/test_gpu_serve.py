@@ -1,0 +1,74 @@
+"""Serving loop: continuous batching, HBM-budget admission, resident rebuilds.
+
+The identity test serves a request stream mixed (several workers, batched
+small systems, resident large systems rebuilt through graph replays) and
+each request alone through the plain single-system call, and requires the
+two pair lists to be equal bit for bit (i, j, S, first); a few small
+requests are also checked against the oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from nlhip import serve, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _stream(n, seed, nmax):
+    return workloads.served_stream(n, rate=1e4, seed=seed, nmin=16, nmax=nmax)
+
+
+def _same(a, b):
+    assert np.array_equal(a.i, b.i) and np.array_equal(a.j, b.j)
+    assert np.array_equal(a.S, b.S) and np.array_equal(a.first, b.first)
+
+
+def test_alone_vs_mixed():
+    reqs = _stream(48, seed=21, nmax=30000)
+    assert any(r.rebuild for r in reqs) and any(r.nat >= 2048 for r in reqs)
+    srv = serve.Server(workers=3, small_atoms=2048, batch_atoms=20000, keep_results=True)
+    mixed = srv.serve(reqs, realtime=False)
+    srv.close()
+    alone = serve.serve_plain(reqs, realtime=False, keep_results=True)
+    assert [c.rid for c in mixed] == [r.rid for r in reqs]
+    assert max(c.batch for c in mixed) > 1                   # small systems were batched
+    for m, a, r in zip(mixed, alone, reqs):
+        print(f"request {r.rid} nat={r.nat} rebuild={r.rebuild}", flush=True)
+        assert m.npairs == a.npairs
+        _same(m.result, a.result)
+    for r, m in list(zip(reqs, mixed))[:12]:
+        if r.nat <= 3000:
+            ref = oracle.neighbour_list(r.X, r.cutoff, r.cell, r.pbc)
+            assert np.array_equal(m.result.j, ref.j) and np.array_equal(m.result.S, ref.S)
+
+
+def test_open_loop_latency_report():
+    reqs = workloads.served_stream(60, rate=2000.0, seed=22, nmax=20000)
+    srv = serve.Server(workers=2, small_atoms=4096)
+    done = srv.serve(reqs, realtime=True)
+    srv.close()
+    s = serve.summarize(done)
+    assert s["requests"] == 60 and s["p99_ms"] >= s["p50_ms"] > 0
+    assert all(c.end >= c.start >= c.arrival - 1e-3 for c in done)
+
+
+def test_admission_under_a_small_budget():
+    # budget for a few systems at a time: everything still completes, the
+    # admitted estimate never exceeds the budget, residents get evicted
+    reqs = _stream(30, seed=23, nmax=40000)
+    budget = max(serve.estimate_bytes(r.nat, r.cell, r.cutoff) for r in reqs) * 2
+    srv = serve.Server(hbm_budget=budget, workers=3, small_atoms=1024, keep_results=True)
+    done = srv.serve(reqs, realtime=False)
+    assert len(done) == 30 and srv.peak_bytes <= budget
+    srv.close()
+    alone = serve.serve_plain(reqs, realtime=False, keep_results=True)
+    for m, a in zip(done, alone):
+        _same(m.result, a.result)
+
+
+def test_request_larger_than_budget_is_refused():
+    reqs = _stream(3, seed=24, nmax=2000)
+    srv = serve.Server(hbm_budget=1 << 20)
+    with pytest.raises(MemoryError):
+        srv.serve(reqs, realtime=False)
