@@ -80,6 +80,33 @@ def _single(s, steps: int, warmup: int, profile: bool = False):
             "ncells": int(g.ncells_total), "phases_ms": phases}
 
 
+def _host_inclusive(s, steps: int) -> dict:
+    """The same build with positions handed over from host memory and i, j,
+    S, first copied back to host arrays (PCIe both ways, inside the timing)."""
+    from nlhip import _lib
+
+    X = np.ascontiguousarray(s.X, dtype=np.float64)
+    g = _lib.geometry(s.cell, s.cutoff, s.pbc, 4)
+    h = _lib.Handle()
+    n = C.c_int64()
+    L = _lib.lib()
+    _lib.check(L.nlh_neighbour_list(h.ptr, X.ctypes.data, X.shape[0], C.byref(g), 0, C.byref(n)))
+    P = int(n.value)
+    i = np.empty(P, np.int32)
+    j = np.empty(P, np.int32)
+    S = np.empty((P, 3), np.int32)
+    first = np.empty(X.shape[0] + 1, np.int32)
+    ts = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        _lib.check(L.nlh_neighbour_list(h.ptr, X.ctypes.data, X.shape[0], C.byref(g), 0, C.byref(n)))
+        _lib.check(L.nlh_copy_pairlist(h.ptr, i.ctypes.data, j.ctypes.data, S.ctypes.data, first.ctypes.data, None))
+        ts.append(time.perf_counter() - t0)
+    t = statistics.median(ts)
+    return {"ms": round(t * 1e3, 4), "pairs_per_s": round(P / t, 1),
+            "note": "positions uploaded from pageable host memory and the PairList copied back, median of steps"}
+
+
 def _roofline(r) -> dict:
     P, nat, M = r["npairs"], r["nat"], r["ncells"]
     b = 20 * P + 32 * nat + 4 * (M + 1)
@@ -195,10 +222,17 @@ def _cpu_baseline(s) -> dict:
         t0 = time.perf_counter()
         P = oracle.time_neighbour_list(s.X, s.cutoff, s.cell, s.pbc, nt)
         ts.append(time.perf_counter() - t0)
+    t1 = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        oracle.time_neighbour_list(s.X, s.cutoff, s.cell, s.pbc, 1)
+        t1.append(time.perf_counter() - t0)
     return {"value": round(P / min(ts), 1), "unit": "pairs/s", "cores": nt, "kind": "port",
             "median_value": round(P / statistics.median(ts), 1),
-            "sample": f"5 full builds of the 100k-atom cube after 1 warm-up (min reported); C oracle, count and "
-                      f"fill over {nt} threads"}
+            "value_1_thread": round(P / min(t1), 1),
+            "sample": f"5 full builds of the 100k-atom cube after 1 warm-up (min reported); C restatement of the "
+                      f"reference CPU sort-based path, count and fill over {nt} threads; value_1_thread: 2 builds "
+                      f"on one thread (min)"}
 
 
 def _copy_calibration(nbytes: int, reps: int) -> dict:
@@ -278,6 +312,7 @@ def main():
         "phases_ms_100k": {k: round(v, 4) for k, v in head["phases_ms"].items()},
     }
     if not args.quick:
+        out["host_inclusive_100k"] = _host_inclusive(c1, max(5, min(args.steps, 20)))
         c1m = workloads.random_cubic(1_000_000, density=0.05, seed=2, cutoff=5.0)
         big = _single(c1m, 10, 3, profile=True)
         out["roofline"] = _roofline(big)

@@ -9,6 +9,16 @@
  * 3x3 mat-vec products and dot products as left-to-right sums of products,
  * ((a1*b1 + a2*b2) + a3*b3), with no fused multiply-add (Julia only fuses
  * under muladd/@fastmath).  Built with -ffp-contract=off.
+ *
+ * What pins each function (tests/test_oracle.py): geometry against numpy's
+ * inv/det/cross and the floor/ceil closed forms; cell ids against
+ * floor(frac*n + 1) with wrap/clamp; the sort and offsets against their
+ * invariants; the pair lists against the reference tests' known answers,
+ * FCC coordination closed forms, each other (sort-based, legacy, brute
+ * force) and scipy's cKDTree.  Parity unpinned at the ulp level against
+ * Julia itself: StaticArrays' exact rounding of inv, lengths, inv_cell'*x
+ * and C'*S cannot be compared without a Julia runtime; R is pinned to 1e-12
+ * by R == X[j] - X[i] + C'*S.
  */
 #include "nl_oracle.h"
 
