@@ -109,7 +109,7 @@ struct nlh_handle {
     int device = 0;
     hipStream_t stream = nullptr;
     DevBuf X, sys, atom_base, cell_base, brick_base, key, slot, cell_count, cell_start, tmp, rec, wbig, perm, cell_id,
-        cell_offsets, pair_count, inv_perm, slots, slotsR, pair_off, partials, sys_pair_base, first, i, j, S, R, status, Xsorted, nb_j, nb_R,
+        cell_offsets, pair_count, fbq, inv_perm, slots, slotsR, pair_off, partials, sys_pair_base, first, i, j, S, R, status, Xsorted, nb_j, nb_R,
         nb_S, nb_count;
     nlh::Status *status_host = nullptr;   // pinned
     void *staging = nullptr;              // pinned upload staging for system tables
@@ -170,6 +170,7 @@ struct nlh_handle {
         b.cell_id_out = cell_id.p;
         b.cell_offsets_out = cell_offsets.p;
         b.pair_count = pair_count.as<int32_t>();
+        b.fb_queue = fbq.as<int32_t>();
         b.inv_perm = inv_perm.as<int32_t>();
         b.slots = slots.p;
         b.slotsR = slotsR.as<double>();
@@ -394,6 +395,7 @@ void assign_bricks(nlh_handle *h)
     }
     h->brick_base_host[nsys] = (int32_t)bricks;
     h->nbricks = bricks;
+    h->fbq.ensure(h->stream, (size_t)std::max<int64_t>(bricks, 1) * 4);
 }
 
 // Upload the system tables (sys, atom/cell/brick bases) through pinned
@@ -602,6 +604,7 @@ void check_pair_buffers(const nlh_handle *h, bool want_R)
     ok = ok && h->pair_count.cap >= n1 * 4 && h->pair_off.cap >= (n1 + 1) * 8 && h->inv_perm.cap >= n1 * 4;
     ok = ok && h->first.cap >= (n1 + h->nsys) * ti && h->sys_pair_base.cap >= (size_t)(h->nsys + 1) * 8;
     ok = ok && h->i.cap >= cap * ti && h->j.cap >= cap * ti && h->S.cap >= cap * 3 * ti;
+    ok = ok && h->fbq.cap >= (size_t)std::max<int64_t>(h->nbricks, 1) * 4;
     if (want_R) ok = ok && h->slotsR.p && h->slotsR.cap >= ent * 24 && h->R.p && h->R.cap >= cap * 24;
     if (!ok) throw Error(NLH_EDEVICE, "internal: pair-stage buffers are smaller than the launch needs");
 }
@@ -886,7 +889,7 @@ int nlh_destroy(nlh_handle *h)
         });
         DevBuf *bufs[] = {&h->X,        &h->sys,       &h->atom_base,  &h->cell_base, &h->brick_base,    &h->key,    &h->slot,
                           &h->cell_count, &h->cell_start, &h->tmp,      &h->rec,          &h->wbig,   &h->perm,
-                          &h->cell_id,  &h->cell_offsets, &h->pair_count, &h->inv_perm, &h->slots, &h->slotsR, &h->pair_off,   &h->partials,
+                          &h->cell_id,  &h->cell_offsets, &h->pair_count, &h->fbq, &h->inv_perm, &h->slots, &h->slotsR, &h->pair_off,   &h->partials,
                           &h->sys_pair_base, &h->first, &h->i,          &h->j,            &h->S,      &h->R,
                           &h->status,   &h->Xsorted,   &h->nb_j,       &h->nb_R,         &h->nb_S,   &h->nb_count};
         for (DevBuf *b : bufs) b->release();

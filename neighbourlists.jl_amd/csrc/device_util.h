@@ -44,18 +44,19 @@ __device__ __forceinline__ uint32_t mbcnt64(uint64_t m)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// Exclusive scan over a 256-thread block; returns this thread's exclusive
+// Exclusive scan over an NT-thread block; returns this thread's exclusive
 // prefix and the block total.
-__device__ __forceinline__ int64_t block_excl_scan(int64_t v, int64_t &total)
+template <int NT>
+__device__ __forceinline__ int64_t block_excl_scan_n(int64_t v, int64_t &total)
 {
-    __shared__ int64_t wsum[kScanThreads / kWave];
+    __shared__ int64_t wsum[NT / kWave];
     const int lane = lane_id(), wv = threadIdx.x / kWave;
     const int64_t inc = wave_incl_scan64(v);
     if (lane == kWave - 1) wsum[wv] = inc;
     __syncthreads();
     int64_t before = 0, tot = 0;
 #pragma unroll
-    for (int w = 0; w < kScanThreads / kWave; ++w) {
+    for (int w = 0; w < NT / kWave; ++w) {
         const int64_t s = wsum[w];
         if (w < wv) before += s;
         tot += s;
@@ -63,6 +64,19 @@ __device__ __forceinline__ int64_t block_excl_scan(int64_t v, int64_t &total)
     __syncthreads();
     total = tot;
     return before + inc - v;
+}
+
+__device__ __forceinline__ int64_t block_excl_scan(int64_t v, int64_t &total)
+{
+    return block_excl_scan_n<kScanThreads>(v, total);
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int l)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)b, l);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(b >> 32), l);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
 // Blocks are dealt to the 8 XCDs round-robin; remap so every XCD works on a

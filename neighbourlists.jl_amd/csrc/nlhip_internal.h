@@ -64,7 +64,8 @@ __host__ __device__ inline void unpack_wind(uint32_t w, int32_t &wx, int32_t &wy
 // Device-side status block, copied to pinned host memory once per build.
 struct Status {
     int64_t npairs;          // total pairs over the batch
-    int64_t reserved;
+    int32_t fb_count;        // bricks queued for the global-memory count path
+    int32_t spare;
     int32_t overflow;        // pair capacity exceeded: outputs not written
     int32_t bad_position;    // non-finite / out-of-range position seen
     int32_t wide;            // a cell shift does not fit a packed slot entry
@@ -165,7 +166,8 @@ struct PipelineBufs {
     void *cell_id_out;          // nat, TI, 1-based
     void *cell_offsets_out;     // M + nsys, TI, 1-based per system
     // pairs
-    int32_t *pair_count;        // nat
+    int32_t *pair_count;
+    int32_t *fb_queue;       // bricks for the global-memory count path (nbricks)
     int32_t *inv_perm;          // nat: original -> sorted index
     void *slots;                // nat * slot_cap slot entries
     double *slotsR;             // 3 * nat * slot_cap (R requested)

@@ -21,6 +21,8 @@
 // fill_kernel -- atoms in their original order, one lane per pair: each wave
 // copies the slots of 64 consecutive atoms into one contiguous range of the
 // PairList arrays (i, j, S and optionally R), so its stores are linear.
+#include <algorithm>
+
 #include "device_util.h"
 
 namespace nlh {
@@ -248,7 +250,7 @@ __device__ __forceinline__ Cand load_candidate(const PairsArgs &A, int k, int to
 // atom, chunk) and append hits to the slots (the brick path's fallback, and
 // the path for neighbour reach beyond one cell).
 template <bool WANT_R, bool WIDE>
-__device__ __noinline__ void count_cell(const PairsArgs &A, const SysDev &sd, int cell, int32_t *__restrict__ pair_count,
+__device__ __forceinline__ void count_cell(const PairsArgs &A, const SysDev &sd, int cell, int32_t *__restrict__ pair_count,
                                         const SlotArgs &O)
 {
     HomeCell H;
@@ -320,12 +322,17 @@ __device__ __noinline__ void count_cell(const PairsArgs &A, const SysDev &sd, in
     }
 }
 
-constexpr int kBrickThreads = 256;
+constexpr int kBrickThreads = 512;                       // 8 waves share one brick
 constexpr int kBrickWaves = kBrickThreads / kWave;
-constexpr int kBrickCap = 2048;   // halo atoms held in LDS
-constexpr int kHaloMax = 512;     // halo cells
+constexpr int kBrickCap = 2048;                          // halo atoms held in LDS
+constexpr int kHaloMax = 512;                            // halo cells
+constexpr int kHaloPerThread = kHaloMax / kBrickThreads;
 constexpr int kBrickUnroll = kBrickCap / kBrickThreads;
-constexpr uint32_t kZeroShiftBit = 1u << 30;
+constexpr int kFallbackThreads = 256;
+constexpr int kFallbackWaves = kFallbackThreads / kWave;
+constexpr uint32_t kPackZero = (512u) | (512u << 10) | (512u << 20);   // pack_wind(0, 0, 0)
+constexpr int kBrickChunks = 3;   // candidate chunks a lane holds in the brick pass (192 candidates)
+static_assert(kHaloPerThread >= 1 && kHaloPerThread * kBrickThreads == kHaloMax, "halo cells per thread");
 
 struct Brick {
     const SysDev *sd;
@@ -333,12 +340,12 @@ struct Brick {
     int nh[3];     // home cells per axis (clipped at the grid edge)
     int hd[3];     // halo extent per axis (nh + 2)
     int nhalo, nhome;
-    bool fast;     // one-cell reach: the LDS path applies
+    bool fast;     // one-cell reach and a small enough halo: the LDS path applies
 };
 
-__device__ __forceinline__ void brick_locate(const PairsArgs &A, const int32_t *__restrict__ sys_brick_base, Brick &B)
+__device__ __forceinline__ void brick_locate(const PairsArgs &A, const int32_t *__restrict__ sys_brick_base, int b,
+                                             Brick &B)
 {
-    const int b = (int)xcd_remap(blockIdx.x, gridDim.x);
     const int s = A.nsys == 1 ? 0 : find_sys(sys_brick_base, A.nsys, (int32_t)b);
     const SysDev &sd = A.sys[s];
     B.sd = &sd;
@@ -363,11 +370,10 @@ __device__ __forceinline__ int brick_cell_id(const Brick &B, int hc)
     return sd.cell_base + (B.b0[0] - 1 + ix) + (B.b0[1] - 1 + iy) * sd.n[0] + (B.b0[2] - 1 + iz) * sd.n[0] * sd.n[1];
 }
 
-template <bool POS>
 struct BrickLDS {
-    double x[POS ? kBrickCap : 1], y[POS ? kBrickCap : 1], z[POS ? kBrickCap : 1];
+    double x[kBrickCap], y[kBrickCap], z[kBrickCap];
     int32_t j[kBrickCap];
-    uint32_t t[kBrickCap];          // pack(s_halo - w_j) | (s_halo == 0) << 30
+    uint32_t t[kBrickCap];          // pack(s_halo - w_j); home cells have s_halo = 0
     int32_t hoff[kHaloMax + 1];     // LDS offset of each halo cell's atoms
     int32_t hstart[kHaloMax];       // first sorted index of each halo cell
     uint32_t hshift[kHaloMax];      // packed cell shift of each halo cell
@@ -375,41 +381,47 @@ struct BrickLDS {
 };
 
 // Stage the brick and its halo in LDS.  Returns false (for every thread of
-// the block) when the brick must take the per-wave path.
-template <bool POS>
-__device__ bool brick_load(const PairsArgs &A, const Brick &B, BrickLDS<POS> &L)
+// the block) when the brick must take the global-memory path.
+__device__ bool brick_load(const PairsArgs &A, const Brick &B, BrickLDS &L)
 {
     const SysDev &sd = *B.sd;
     const int tid = threadIdx.x;
     const bool p[3] = {(sd.pbc & 1) != 0, (sd.pbc & 2) != 0, (sd.pbc & 4) != 0};
-    // halo cell table: thread tid owns halo cells 2 tid and 2 tid + 1
-    int cnt[2] = {0, 0};
+    // halo cell table: thread tid owns halo cells kHaloPerThread * tid + u
+    int cnt[kHaloPerThread];
+    int sum = 0;
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-        const int h = 2 * tid + u;
+    for (int u = 0; u < kHaloPerThread; ++u) {
+        cnt[u] = 0;
+        const int h = kHaloPerThread * tid + u;
         if (h < B.nhalo) {
             const int ux = h % B.hd[0], uy = (h / B.hd[0]) % B.hd[1], uz = h / (B.hd[0] * B.hd[1]);
             const int a[3] = {B.b0[0] - 1 + ux, B.b0[1] - 1 + uy, B.b0[2] - 1 + uz};
             bool inb = true;
             for (int k = 0; k < 3; ++k) inb = inb && (p[k] || (a[k] >= 1 && a[k] <= sd.n[k]));
             int st = 0;
-            uint32_t sh = 0;
-            if (inb) {
-                int32_t c[3], s[3];
-                for (int k = 0; k < 3; ++k) wrap_and_shift(a[k], sd.n[k], p[k], c[k], s[k]);
+            uint32_t sh = kPackZero;
+            if (inb) {   // _is_cell_in_bounds / _get_neighbor_cell (gpu_kernels.jl:20-47)
+                int32_t c[3], sft[3];
+                for (int k = 0; k < 3; ++k) wrap_and_shift(a[k], sd.n[k], p[k], c[k], sft[k]);
                 const int gc = sd.cell_base + (c[0] - 1) + (c[1] - 1) * sd.n[0] + (c[2] - 1) * sd.n[0] * sd.n[1];
                 st = A.cell_start[gc];
                 cnt[u] = A.cell_start[gc + 1] - st;
-                sh = pack_wind(s[0], s[1], s[2]);
+                sh = pack_wind(sft[0], sft[1], sft[2]);
             }
             L.hstart[h] = st;
             L.hshift[h] = sh;
         }
+        sum += cnt[u];
     }
     int64_t tot;
-    const int ex = (int)block_excl_scan(cnt[0] + cnt[1], tot);
-    if (2 * tid < B.nhalo) L.hoff[2 * tid] = ex;
-    if (2 * tid + 1 < B.nhalo) L.hoff[2 * tid + 1] = ex + cnt[0];
+    int run = (int)block_excl_scan_n<kBrickThreads>(sum, tot);
+#pragma unroll
+    for (int u = 0; u < kHaloPerThread; ++u) {
+        const int h = kHaloPerThread * tid + u;
+        if (h < B.nhalo) L.hoff[h] = run;
+        run += cnt[u];
+    }
     if (tid == 0) {
         L.hoff[B.nhalo] = (int32_t)tot;
         L.total = (int32_t)tot;
@@ -425,7 +437,7 @@ __device__ bool brick_load(const PairsArgs &A, const Brick &B, BrickLDS<POS> &L)
     for (int u = 0; u < kBrickUnroll; ++u) {
         const int idx = tid + u * kBrickThreads;
         pg[u] = 0;
-        hs[u] = 0;
+        hs[u] = kPackZero;
         if (idx < total) {
             int lo = 0, hi = B.nhalo - 1;   // largest h with hoff[h] <= idx
             while (lo < hi) {
@@ -450,37 +462,36 @@ __device__ bool brick_load(const PairsArgs &A, const Brick &B, BrickLDS<POS> &L)
             unpack_wind(hs[u], sx, sy, sz);
             const uint32_t t = pack_wind(sx - wx, sy - wy, sz - wz);
             if (t & kWindBig) L.flag = 1;
-            L.t[idx] = t | ((sx | sy | sz) == 0 ? kZeroShiftBit : 0u);
+            L.t[idx] = t;
             L.j[idx] = r[u].j;
-            if (POS) {
-                L.x[idx] = r[u].x;
-                L.y[idx] = r[u].y;
-                L.z[idx] = r[u].z;
-            }
+            L.x[idx] = r[u].x;
+            L.y[idx] = r[u].y;
+            L.z[idx] = r[u].z;
         }
     }
     __syncthreads();
     return L.flag == 0;
 }
 
-// Per home cell of a brick: its LDS atom range, first sorted index and the
-// nine candidate rows (lane r < 9 holds row r).  Returns the candidate count.
+// Per home cell of a brick: its LDS atom range, first sorted index, the
+// nine candidate rows (lane r < 9 holds row r) and the candidate index of
+// the home cell's first atom (its own self candidate).
 struct HomeRows {
-    int h0, h1;      // LDS range of the home atoms
-    int pbase;       // sorted index of the first home atom
+    int h0, h1;      // LDS range of the home atoms (uniform)
+    int pbase;       // sorted index of the first home atom (uniform)
     int rstart, rex; // per lane: LDS start and exclusive candidate prefix of row `lane`
-    int total;
+    int total;       // candidates (uniform)
+    int self0;       // candidate index of home atom 0 (uniform)
 };
 
-template <bool POS>
-__device__ __forceinline__ HomeRows brick_home(const Brick &B, const BrickLDS<POS> &L, int hc)
+__device__ __forceinline__ HomeRows brick_home(const Brick &B, const BrickLDS &L, int hc)
 {
     HomeRows R;
     const int ix = hc % B.nh[0], iy = (hc / B.nh[0]) % B.nh[1], iz = hc / (B.nh[0] * B.nh[1]);
     const int hh = (ix + 1) + B.hd[0] * ((iy + 1) + B.hd[1] * (iz + 1));
-    R.h0 = L.hoff[hh];
-    R.h1 = L.hoff[hh + 1];
-    R.pbase = L.hstart[hh];
+    R.h0 = sgpr(L.hoff[hh]);
+    R.h1 = sgpr(L.hoff[hh + 1]);
+    R.pbase = sgpr(L.hstart[hh]);
     const int lane = lane_id();
     int len = 0, st = 0;
     if (lane < 9) {   // row (dz, dy) = (lane / 3 - 1, lane % 3 - 1); dx = -1..1 contiguous
@@ -493,6 +504,8 @@ __device__ __forceinline__ HomeRows brick_home(const Brick &B, const BrickLDS<PO
     R.rex = incl - len;
     R.rstart = st;
     R.total = sgpr(__shfl(incl, kWave - 1, kWave));
+    // row 4 is (dz, dy) = (0, 0); the home cell is its middle cell
+    R.self0 = __builtin_amdgcn_readlane(R.rex, 4) + (R.h0 - __builtin_amdgcn_readlane(R.rstart, 4));
     return R;
 }
 
@@ -508,114 +521,178 @@ __device__ __forceinline__ int row_index(const HomeRows &R, int k)
     return __shfl(R.rstart, m, kWave) + (k - __shfl(R.rex, m, kWave));
 }
 
+// Append one ballot's hits to a home atom's slot row.  `row` and `rrow` are
+// the (wave-uniform) slot row and its R row; ca the atom's running count;
+// ent = j | pack(s - w_j) << 32 is the candidate's entry when w_i = 0.
+template <bool WANT_R, bool WIDE>
+__device__ __forceinline__ void emit_row(void *__restrict__ row, double *__restrict__ rrow, uint32_t cap,
+                                         Status *__restrict__ status, bool hit, uint64_t m, int32_t &ca, uint64_t ent,
+                                         bool wzero, int32_t wix, int32_t wiy, int32_t wiz, double Rx, double Ry,
+                                         double Rz)
+{
+    const uint32_t r = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, (uint32_t)ca));
+    if (hit && r < cap) {
+        if (WIDE) {
+            int32_t tx, ty, tz;
+            unpack_wind((uint32_t)(ent >> 32), tx, ty, tz);
+            reinterpret_cast<int4 *>(row)[r] = make_int4((int32_t)(uint32_t)ent, tx + wix, ty + wiy, tz + wiz);
+        } else if (wzero) {
+            // S = s_loop + w_i - w_j = s_loop - w_j  (gpu_kernels.jl:88)
+            reinterpret_cast<uint64_t *>(row)[r] = ent;
+        } else {
+            int32_t tx, ty, tz;
+            unpack_wind((uint32_t)(ent >> 32), tx, ty, tz);
+            const uint32_t spk = pack_wind(tx + wix, ty + wiy, tz + wiz);
+            if (spk & kWindBig) status->wide = 1;
+            reinterpret_cast<uint64_t *>(row)[r] = (ent & 0xffffffffull) | ((uint64_t)spk << 32);
+        }
+        if (WANT_R) {
+            double *Rd = rrow + 3 * r;
+            Rd[0] = Rx;
+            Rd[1] = Ry;
+            Rd[2] = Rz;
+        }
+    }
+    ca += __popcll(m);
+}
 
+// Brick pass of the count: home atoms stream through the wave as scalars
+// (read out of one-per-lane registers), candidates sit one per lane in
+// registers, chunk by chunk; each (home atom, chunk) is one FP64 distance
+// test and one ballot.  Bricks that cannot take this path are queued for
+// count_fallback_kernel.
 template <bool WANT_R, bool WIDE>
 __global__ __launch_bounds__(kBrickThreads) void count_kernel(
     const SysDev *__restrict__ sys, const int32_t *__restrict__ sys_cell_base, const int32_t *__restrict__ sys_brick_base,
     int nsys, int32_t M, int64_t nat, const int32_t *__restrict__ cell_start, const Rec *__restrict__ rec,
     const int32_t *__restrict__ wbig, int32_t *__restrict__ pair_count, void *__restrict__ slots,
-    double *__restrict__ slotsR, int32_t cap, Status *__restrict__ status)
+    double *__restrict__ slotsR, int32_t cap, Status *__restrict__ status, int32_t *__restrict__ fb_queue)
 {
-    __shared__ BrickLDS<true> L;
+    __shared__ BrickLDS L;
     __shared__ double Csh[9];
     const PairsArgs A{sys, sys_cell_base, nsys, M, nat, cell_start, rec, wbig};
-    const SlotArgs O{slots, slotsR, cap, status};
+    const int b = (int)xcd_remap(blockIdx.x, gridDim.x);
     Brick B;
-    brick_locate(A, sys_brick_base, B);
+    brick_locate(A, sys_brick_base, b, B);
     if (threadIdx.x < 9) Csh[threadIdx.x] = B.sd->C[threadIdx.x];   // visible after brick_load's barrier
-    const int wv = sgpr((int)(threadIdx.x >> 6));
-    const bool lds = B.fast && brick_load<true>(A, B, L);
+    const bool lds = B.fast && brick_load(A, B, L);
     if (!lds) {
-        for (int hc = wv; hc < B.nhome; hc += kBrickWaves)
-            count_cell<WANT_R, WIDE>(A, *B.sd, brick_cell_id(B, hc), pair_count, O);
+        if (threadIdx.x == 0) fb_queue[atomicAdd(&status->fb_count, 1)] = b;
         return;
     }
     const SysDev &sd = *B.sd;
     const double rc2 = sd.rc2;
     const int lane = lane_id();
+    const int wv = sgpr((int)(threadIdx.x >> 6));
     for (int hc = wv; hc < B.nhome; hc += kBrickWaves) {
         const HomeRows R = brick_home(B, L, hc);
         const int nha = R.h1 - R.h0;
         for (int hg = 0; hg < nha; hg += kWave) {
             const int nh = sgpr(min(kWave, nha - hg));
-            const int32_t hj_lane = lane < nh ? L.j[R.h0 + hg + lane] : 0;
-            const int64_t p0 = (int64_t)R.pbase + hg;   // sorted index of home atom 0 of the group
-            int32_t cnt_lane = 0;                        // lane a: running count of home atom a
-            for (int k0 = 0; k0 < R.total; k0 += kChunksPerPass * kWave) {
-                const int nch = sgpr(min(kChunksPerPass, (R.total - k0 + kWave - 1) / kWave));
-                double cxp[kChunksPerPass], cyp[kChunksPerPass], czp[kChunksPerPass];
-                // key: j where the candidate's cell shift is zero, else -3 (never
-                // a home index), so (key == j_i) is _is_self_interaction
-                int32_t cj[kChunksPerPass], key[kChunksPerPass];
-                uint32_t tpk[kChunksPerPass];   // packed s - w_j
-                double sx[kChunksPerPass], sy[kChunksPerPass], sz[kChunksPerPass];
+            const int hl = R.h0 + hg + (lane < nh ? lane : 0);
+            const double hx_l = L.x[hl], hy_l = L.y[hl], hz_l = L.z[hl];
+            const uint32_t ht_l = L.t[hl];   // pack(-w_i): a home cell's own shift is zero
+            const int32_t hj_l = L.j[hl];
+            const int64_t p0 = (int64_t)R.pbase + hg;
+            const int kself = R.self0 + hg;  // candidate index of home atom hg + a is kself + a
+            int32_t cnt_l = 0;                // lane a: running count of home atom hg + a
+            for (int k0 = 0; k0 < R.total; k0 += kBrickChunks * kWave) {
+                const int nch = sgpr(min(kBrickChunks, (R.total - k0 + kWave - 1) / kWave));
+                double cx[kBrickChunks], cy[kBrickChunks], cz[kBrickChunks];
+                double sx[kBrickChunks], sy[kBrickChunks], sz[kBrickChunks];
+                uint64_t ent[kBrickChunks];   // j | pack(s - w_j) << 32
 #pragma unroll
-                for (int g = 0; g < kChunksPerPass; ++g) {
-                    cxp[g] = cyp[g] = czp[g] = kFar;
-                    sx[g] = sy[g] = sz[g] = 0.0;
-                    cj[g] = -1;
-                    key[g] = -3;
-                    tpk[g] = pack_wind(0, 0, 0);
+                for (int g = 0; g < kBrickChunks; ++g) {
+                    cx[g] = cy[g] = cz[g] = kFar;
+                    ent[g] = (uint64_t)kPackZero << 32;
                     if (g < nch) {
                         const int k = k0 + g * kWave + lane;
                         const bool v = k < R.total;
                         const int li = row_index(R, v ? k : 0);
                         if (v) {
-                            cxp[g] = L.x[li];
-                            cyp[g] = L.y[li];
-                            czp[g] = L.z[li];
-                            cj[g] = L.j[li];
-                            const uint32_t t = L.t[li];
-                            key[g] = (t & kZeroShiftBit) ? cj[g] : -3;
-                            tpk[g] = t & ~kZeroShiftBit;
+                            cx[g] = L.x[li];
+                            cy[g] = L.y[li];
+                            cz[g] = L.z[li];
+                            ent[g] = (uint64_t)(uint32_t)L.j[li] | ((uint64_t)L.t[li] << 32);
                         }
                     }
                 }
-                uint32_t wcache = 0xffffffffu;
+                uint32_t wcur = 0xffffffffu;   // home winding the shifts were computed for
                 int32_t wix = 0, wiy = 0, wiz = 0;
                 for (int a = 0; a < nh; ++a) {
-                    const int ha = R.h0 + hg + a;   // uniform LDS address
-                    const double hx = L.x[ha], hy = L.y[ha], hz = L.z[ha];
-                    const int32_t hj = L.j[ha];
-                    const uint32_t ht = (uint32_t)sgpr((int)(L.t[ha] & ~kZeroShiftBit));   // t = -w_i (home shift 0)
-                    if (ht != wcache) {
+                    const double hx = readlane_f64(hx_l, a), hy = readlane_f64(hy_l, a), hz = readlane_f64(hz_l, a);
+                    const uint32_t ht = (uint32_t)__builtin_amdgcn_readlane((int)ht_l, a);
+                    if (ht != wcur) {   // first home atom, or a new home winding
                         int32_t tx, ty, tz;
                         unpack_wind(ht, tx, ty, tz);
                         wix = -tx; wiy = -ty; wiz = -tz;
 #pragma unroll
-                        for (int g = 0; g < kChunksPerPass; ++g) {
+                        for (int g = 0; g < kBrickChunks; ++g) {
+                            sx[g] = sy[g] = sz[g] = 0.0;
                             if (g < nch) {
-                                int32_t cx, cy, cz;
-                                unpack_wind(tpk[g], cx, cy, cz);
-                                cell_shift(Csh, cx + wix, cy + wiy, cz + wiz, sx[g], sy[g], sz[g]);
+                                int32_t cx_, cy_, cz_;
+                                unpack_wind((uint32_t)(ent[g] >> 32), cx_, cy_, cz_);
+                                cell_shift(Csh, cx_ + wix, cy_ + wiy, cz_ + wiz, sx[g], sy[g], sz[g]);
                             }
                         }
-                        wcache = ht;
+                        wcur = ht;
                     }
-                    const bool wzero = (wix | wiy | wiz) == 0;
-                    int32_t ca = __builtin_amdgcn_readlane(cnt_lane, a);
-                    const int64_t sbase = (p0 + a) * (int64_t)cap;
+                    const bool wzero = ht == kPackZero;
+                    int32_t ca = __builtin_amdgcn_readlane(cnt_l, a);
+                    const int64_t srow = (p0 + a) * (int64_t)cap;
+                    void *row = WIDE ? (void *)(reinterpret_cast<int4 *>(slots) + srow)
+                                     : (void *)(reinterpret_cast<uint64_t *>(slots) + srow);
+                    double *rrow = WANT_R ? slotsR + 3 * srow : nullptr;
+                    const int ks = kself + a - k0;   // the self candidate's place in this pass
 #pragma unroll
-                    for (int g = 0; g < kChunksPerPass; ++g) {
+                    for (int g = 0; g < kBrickChunks; ++g) {
                         if (g < nch) {
                             // R = xj - xi + C' * S ; r^2 = dot(R, R)  (gpu_kernels.jl:89-92)
-                            const double Rx = (cxp[g] - hx) + sx[g];
-                            const double Ry = (cyp[g] - hy) + sy[g];
-                            const double Rz = (czp[g] - hz) + sz[g];
+                            const double Rx = (cx[g] - hx) + sx[g];
+                            const double Ry = (cy[g] - hy) + sy[g];
+                            const double Rz = (cz[g] - hz) + sz[g];
                             const double r2 = (Rx * Rx + Ry * Ry) + Rz * Rz;
-                            const uint64_t mask = __ballot((r2 < rc2) & (key[g] != hj));
-                            int32_t tx = 0, ty = 0, tz = 0;
-                            if (WIDE || !wzero) unpack_wind(tpk[g], tx, ty, tz);
-                            emit_hits<WANT_R, WIDE>(O, mask, sbase, ca, cj[g], tpk[g], tx, ty, tz, wix, wiy, wiz, wzero,
-                                                    Rx, Ry, Rz);
+                            // _is_self_interaction: j == i with no cell shift is the home
+                            // atom's own slot in the home cell, lane ks - 64 g of this chunk
+                            const bool hit = (r2 < rc2) & (lane != ks - g * kWave);
+                            const uint64_t m = __builtin_amdgcn_ballot_w64(hit);
+                            if (m) emit_row<WANT_R, WIDE>(row, rrow, (uint32_t)cap, status, hit, m, ca, ent[g], wzero,
+                                                          wix, wiy, wiz, Rx, Ry, Rz);
                         }
                     }
-                    cnt_lane = lane == a ? ca : cnt_lane;
+                    cnt_l = lane == a ? ca : cnt_l;
                 }
             }
-            if (lane < nh) pair_count[sd.atom_base + hj_lane] = cnt_lane;
+            if (lane < nh) pair_count[sd.atom_base + hj_l] = cnt_l;
         }
     }
+}
+
+// Global-memory count of the bricks the brick pass queued (reach beyond one
+// cell, halo over the LDS budget, windings beyond the packed range): one
+// home cell per wave.
+template <bool WANT_R, bool WIDE>
+__global__ __launch_bounds__(kFallbackThreads) void count_fallback_kernel(
+    const SysDev *__restrict__ sys, const int32_t *__restrict__ sys_cell_base, const int32_t *__restrict__ sys_brick_base,
+    int nsys, int32_t M, int64_t nat, const int32_t *__restrict__ cell_start, const Rec *__restrict__ rec,
+    const int32_t *__restrict__ wbig, int32_t *__restrict__ pair_count, void *__restrict__ slots,
+    double *__restrict__ slotsR, int32_t cap, Status *__restrict__ status, const int32_t *__restrict__ fb_queue)
+{
+    const int nq = status->fb_count;
+    const PairsArgs A{sys, sys_cell_base, nsys, M, nat, cell_start, rec, wbig};
+    const SlotArgs O{slots, slotsR, cap, status};
+    const int wv = sgpr((int)(threadIdx.x >> 6));
+    for (int q = blockIdx.x; q < nq; q += gridDim.x) {
+        Brick B;
+        brick_locate(A, sys_brick_base, fb_queue[q], B);
+        for (int hc = wv; hc < B.nhome; hc += kFallbackWaves)
+            count_cell<WANT_R, WIDE>(A, *B.sd, brick_cell_id(B, hc), pair_count, O);
+    }
+}
+
+__global__ void reset_queue_kernel(Status *__restrict__ status)
+{
+    if (threadIdx.x == 0) status->fb_count = 0;
 }
 
 // Fill pass: atoms in their original order, one wave per 64 consecutive
@@ -684,9 +761,14 @@ __global__ __launch_bounds__(1024) void fill_kernel(
 template <bool R, bool W>
 static void count_launch(const PipelineBufs &b, hipStream_t st)
 {
+    hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, st, b.status);
     hipLaunchKernelGGL((count_kernel<R, W>), dim3((unsigned)b.nbricks), dim3(kBrickThreads), 0, st, b.sys,
                        b.sys_cell_base, b.sys_brick_base, b.nsys, b.M, b.nat, b.cell_start, b.rec, b.wbig,
-                       b.pair_count, b.slots, b.slotsR, b.slot_cap, b.status);
+                       b.pair_count, b.slots, b.slotsR, b.slot_cap, b.status, b.fb_queue);
+    const unsigned nfb = (unsigned)std::min<int64_t>(b.nbricks, 1024);
+    hipLaunchKernelGGL((count_fallback_kernel<R, W>), dim3(nfb), dim3(kFallbackThreads), 0, st, b.sys,
+                       b.sys_cell_base, b.sys_brick_base, b.nsys, b.M, b.nat, b.cell_start, b.rec, b.wbig,
+                       b.pair_count, b.slots, b.slotsR, b.slot_cap, b.status, (const int32_t *)b.fb_queue);
 }
 
 void launch_count(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st)
