@@ -144,7 +144,7 @@ struct nlh_handle {
     nlh::PipelineBufs graph_bufs;
     int64_t graph_launches = 0;
     // launch configuration (nlh_set_tuning / nlh_autotune)
-    nlh_tuning tune = {nlh::kBrickAtomBudget, 256};
+    nlh_tuning tune = {nlh::kBrickAtomBudget, 256, 8};
 
     nlh::PipelineBufs bufs() const
     {
@@ -187,6 +187,7 @@ struct nlh_handle {
         b.status = status.as<nlh::Status>();
         b.ti_bytes = ti_bytes;
         b.fill_block = tune.fill_block;
+        b.fill_atoms = tune.fill_atoms;
         return b;
     }
 };
@@ -357,24 +358,32 @@ void drop_graph(nlh_handle *h)
 // Pair-search brick of a system: the largest of a fixed ladder of shapes
 // whose halo is expected to fit the LDS budget at this system's mean atom
 // density per cell (clipped to the cell grid).
-void choose_brick(const nlh_geometry &g, int64_t nat, int32_t budget, int32_t out[3])
+void choose_brick(const nlh_geometry &g, int64_t nat, int32_t budget, int32_t out[3], int64_t min_bricks = 0)
 {
     static const int32_t ladder[][3] = {{4, 4, 4}, {4, 4, 2}, {4, 2, 2}, {2, 2, 2}, {2, 2, 1}, {2, 1, 1}, {1, 1, 1}};
     const double per_cell = g.ncells_total > 0 ? (double)nat / (double)g.ncells_total : 0.0;
+    bool found = false;
     for (const auto &b : ladder) {
         int32_t d[3];
         double halo = 1.0;
+        int64_t nb = 1;
         for (int k = 0; k < 3; ++k) {
             d[k] = (int32_t)std::min<int64_t>(b[k], g.ncells[k]);
             halo *= (double)(d[k] + 2);
+            nb *= (g.ncells[k] + d[k] - 1) / d[k];
         }
         if (halo * per_cell * 1.25 <= (double)budget && halo <= 512.0) {
+            // the first shape that fits; a smaller one while the grid is too
+            // coarse to fill the device, down to one home cell per wave
             std::memcpy(out, d, sizeof d);
-            return;
+            found = true;
+            if (nb >= min_bricks || (int64_t)d[0] * d[1] * d[2] <= 8) return;
         }
     }
-    out[0] = out[1] = out[2] = 1;
+    if (!found) out[0] = out[1] = out[2] = 1;
 }
+
+constexpr int64_t kMinBricks = 1024;
 
 // Pair-search bricks of every system for the current brick budget: brick
 // shape per system and the first brick of each system (host tables; the
@@ -386,7 +395,9 @@ void assign_bricks(nlh_handle *h)
     h->brick_base_host.resize(nsys + 1);
     for (int s = 0; s < nsys; ++s) {
         nlh::SysDev &sd = h->sys_host[s];
-        choose_brick(h->geoms[s], sd.nat, h->tune.brick_atoms, sd.brick);
+        // a single system splits into at least two rounds of bricks over the
+        // device (2 bricks per CU fit its LDS)
+        choose_brick(h->geoms[s], sd.nat, h->tune.brick_atoms, sd.brick, nsys == 1 ? kMinBricks : 0);
         sd.brick_base = (int32_t)bricks;
         h->brick_base_host[s] = (int32_t)bricks;
         bricks += (int64_t)((sd.n[0] + sd.brick[0] - 1) / sd.brick[0]) * ((sd.n[1] + sd.brick[1] - 1) / sd.brick[1]) *
@@ -759,6 +770,8 @@ void validate_tuning(const nlh_tuning &t)
         throw Error(NLH_EINVAL, "brick_atoms must be in [64, 2048] (the LDS holds 2048 halo atoms)");
     if (t.fill_block < 64 || t.fill_block > 1024 || t.fill_block % 64 != 0)
         throw Error(NLH_EINVAL, "fill_block must be a multiple of 64 in [64, 1024]");
+    if (t.fill_atoms != 4 && t.fill_atoms != 8 && t.fill_atoms != 16 && t.fill_atoms != 32 && t.fill_atoms != 64)
+        throw Error(NLH_EINVAL, "fill_atoms must be 4, 8, 16, 32 or 64");
 }
 
 void apply_tuning(nlh_handle *h, const nlh_tuning &t)
@@ -1201,9 +1214,11 @@ int nlh_autotune(nlh_handle *h, const double *X_batch, uint32_t flags, int32_t r
         try {
             static const int32_t bricks[] = {512, 1024, 1536};
             static const int32_t fills[] = {128, 256, 512};
+            static const int32_t atoms[] = {4, 8, 16};
             for (int32_t ba : bricks)
-                for (int32_t fb : fills) {
-                    apply_tuning(h, nlh_tuning{ba, fb});
+                for (int32_t fb : fills)
+                    for (int32_t fa : atoms) {
+                    apply_tuning(h, nlh_tuning{ba, fb, fa});
                     rebuild_eager(h, flags & ~NLH_NO_SYNC);   // warm-up (and buffer growth)
                     float tot = 0.0f;
                     for (int r = 0; r < reps; ++r) {

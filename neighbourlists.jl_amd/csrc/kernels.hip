@@ -41,7 +41,8 @@ __global__ __launch_bounds__(256) void bin_kernel(const double *__restrict__ X, 
 // 2. scans (reduce -> scan partials -> downsweep)
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ void load_tile(const int32_t *__restrict__ in, int64_t n, int64_t base, int32_t v[kScanItems])
+__device__ __forceinline__ void load_tile(const int32_t *__restrict__ in, int64_t n, int64_t base, int32_t v[kScanItems],
+                                          int32_t mask = -1)
 {
     const int64_t b = base + (int64_t)threadIdx.x * kScanItems;
     if (b + kScanItems <= n) {
@@ -49,19 +50,19 @@ __device__ __forceinline__ void load_tile(const int32_t *__restrict__ in, int64_
 #pragma unroll
         for (int q = 0; q < kScanItems / 4; ++q) {
             const int4 t = p[q];
-            v[4 * q] = t.x; v[4 * q + 1] = t.y; v[4 * q + 2] = t.z; v[4 * q + 3] = t.w;
+            v[4 * q] = t.x & mask; v[4 * q + 1] = t.y & mask; v[4 * q + 2] = t.z & mask; v[4 * q + 3] = t.w & mask;
         }
     } else {
 #pragma unroll
-        for (int q = 0; q < kScanItems; ++q) v[q] = (b + q < n) ? in[b + q] : 0;
+        for (int q = 0; q < kScanItems; ++q) v[q] = (b + q < n) ? in[b + q] & mask : 0;
     }
 }
 
 __global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const int32_t *__restrict__ in, int64_t n,
-                                                                   int64_t *__restrict__ partials)
+                                                                   int64_t *__restrict__ partials, int32_t mask)
 {
     int32_t v[kScanItems];
-    load_tile(in, n, (int64_t)blockIdx.x * kScanTile, v);
+    load_tile(in, n, (int64_t)blockIdx.x * kScanTile, v, mask);
     int64_t s = 0;
 #pragma unroll
     for (int q = 0; q < kScanItems; ++q) s += v[q];
@@ -136,7 +137,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_pairs_down_kernel(
 {
     int32_t v[kScanItems];
     const int64_t base = (int64_t)blockIdx.x * kScanTile;
-    load_tile(counts, nat, base, v);
+    load_tile(counts, nat, base, v, kCountMask);   // bit 30: slot-row format
     int64_t s = 0;
 #pragma unroll
     for (int q = 0; q < kScanItems; ++q) s += v[q];
@@ -295,7 +296,7 @@ void launch_build(const PipelineBufs &b, hipStream_t st)
     const int64_t nb = (b.M + kScanTile - 1) / kScanTile;
     if (nb > 1) {
         hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, st, b.cell_count,
-                           (int64_t)b.M, b.scan_partials);
+                           (int64_t)b.M, b.scan_partials, (int32_t)-1);
         hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(kScanThreads), 0, st, b.scan_partials, (int)nb,
                            (Status *)nullptr, (int64_t)0);
     }
@@ -319,7 +320,7 @@ void launch_scan_pairs(const PipelineBufs &b, hipStream_t st)
     const int64_t nb = n > 0 ? (n + kScanTile - 1) / kScanTile : 1;
     if (nb > 1) {
         hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, st, b.pair_count, n,
-                           b.scan_partials);
+                           b.scan_partials, kCountMask);
         hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(kScanThreads), 0, st, b.scan_partials, (int)nb,
                            b.status, b.pair_capacity);
     }

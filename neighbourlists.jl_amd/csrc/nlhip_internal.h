@@ -48,6 +48,23 @@ struct __attribute__((aligned(16))) Rec {
 
 constexpr uint32_t kWindBig = 0x80000000u;
 
+// Per-atom neighbour counts carry the format of the atom's slot row in bit
+// 30: clear = 4-byte entries (j | S code << 26, every |S_k| <= 1 and j < 2^26),
+// set = 8-byte entries (j | packed S << 32).
+constexpr int32_t kRow8Bit = 1 << 30;
+constexpr int32_t kCountMask = kRow8Bit - 1;
+constexpr int32_t kNarrowMaxAtoms = 1 << 26;
+
+// 2-bit code of a shift component in {-1, 0, 1} and back
+__host__ __device__ inline uint32_t narrow_code(int32_t sx, int32_t sy, int32_t sz)
+{
+    return ((uint32_t)sx & 3u) | (((uint32_t)sy & 3u) << 2) | (((uint32_t)sz & 3u) << 4);
+}
+__host__ __device__ inline int32_t narrow_comp(uint32_t code, int k)
+{
+    return ((int32_t)(code << (30 - 2 * k))) >> 30;
+}
+
 __host__ __device__ inline uint32_t pack_wind(int32_t wx, int32_t wy, int32_t wz)
 {
     if (wx < -512 || wx > 511 || wy < -512 || wy > 511 || wz < -512 || wz > 511) return kWindBig;
@@ -182,6 +199,7 @@ struct PipelineBufs {
     Status *status;
     int ti_bytes;
     int fill_block;          // threads per fill workgroup (multiple of 64)
+    int fill_atoms;          // atoms per fill wave (16, 32 or 64)
 };
 
 void launch_clear_status(Status *status, hipStream_t st);

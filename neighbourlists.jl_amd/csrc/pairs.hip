@@ -76,9 +76,11 @@ struct PairsArgs {
 };
 
 // ---------------------------------------------------------------------------
-// slots: each atom's neighbours (j, S) in traversal order, `cap` per atom,
-// indexed by sorted atom position.  Narrow entries pack S into 3 x 10 bits
-// (8 bytes); wide entries keep four int32.
+// slots: each atom's neighbours (j, S) in traversal order, a row of `cap`
+// entries per atom, rows in original atom order.  A row holds 4-byte entries
+// (j | S code << 26) when every shift of its brick is within [-1, 1]^3, else
+// 8-byte entries (j | S packed in 3 x 10 bits << 32); bit 30 of the atom's
+// count tells which.  Wide builds (shifts beyond +-511) use 16-byte entries.
 // ---------------------------------------------------------------------------
 
 template <bool WIDE>
@@ -301,7 +303,7 @@ __device__ __forceinline__ void count_cell(const PairsArgs &A, const SysDev &sd,
                     }
                     const bool wzero = (wix | wiy | wiz) == 0;
                     int32_t ca = __builtin_amdgcn_readlane(cnt_lane, a);
-                    const int64_t sbase = (int64_t)(hg + a) * O.cap;
+                    const int64_t sbase = (H.abase + __builtin_amdgcn_readlane(hj_lane, a)) * (int64_t)O.cap;
 #pragma unroll
                     for (int g = 0; g < kChunksPerPass; ++g) {
                         if (g < nch) {
@@ -318,7 +320,7 @@ __device__ __forceinline__ void count_cell(const PairsArgs &A, const SysDev &sd,
                 }
             }
         }
-        if (lane < nh) pair_count[H.abase + hj_lane] = cnt_lane;
+        if (lane < nh) pair_count[H.abase + hj_lane] = cnt_lane | kRow8Bit;
     }
 }
 
@@ -376,9 +378,12 @@ struct BrickLDS {
     uint32_t t[kBrickCap];          // pack(s_halo - w_j); home cells have s_halo = 0
     int32_t hoff[kHaloMax + 1];     // LDS offset of each halo cell's atoms
     int32_t hstart[kHaloMax];       // first sorted index of each halo cell
-    uint32_t hshift[kHaloMax];      // packed cell shift of each halo cell
+    uint32_t hshift[kHaloMax];      // packed cell shift of each halo cell | kHomeCell
     int32_t total, flag;
+    int32_t row8;                   // some shift needs 8-byte slot entries
 };
+
+constexpr uint32_t kHomeCell = 1u << 30;   // halo-table mark of the brick's own cells
 
 // Stage the brick and its halo in LDS.  Returns false (for every thread of
 // the block) when the brick must take the global-memory path.
@@ -409,8 +414,9 @@ __device__ bool brick_load(const PairsArgs &A, const Brick &B, BrickLDS &L)
                 cnt[u] = A.cell_start[gc + 1] - st;
                 sh = pack_wind(sft[0], sft[1], sft[2]);
             }
+            const bool home = ux >= 1 && ux <= B.nh[0] && uy >= 1 && uy <= B.nh[1] && uz >= 1 && uz <= B.nh[2];
             L.hstart[h] = st;
-            L.hshift[h] = sh;
+            L.hshift[h] = sh | (home ? kHomeCell : 0u);
         }
         sum += cnt[u];
     }
@@ -426,6 +432,7 @@ __device__ bool brick_load(const PairsArgs &A, const Brick &B, BrickLDS &L)
         L.hoff[B.nhalo] = (int32_t)tot;
         L.total = (int32_t)tot;
         L.flag = 0;
+        L.row8 = sd.nat > kNarrowMaxAtoms ? 1 : 0;
     }
     __syncthreads();
     const int total = L.total;
@@ -459,9 +466,16 @@ __device__ bool brick_load(const PairsArgs &A, const Brick &B, BrickLDS &L)
         if (idx < total) {
             int32_t wx, wy, wz, sx, sy, sz;
             rec_winding(r[u], A.wbig, pg[u], wx, wy, wz);
-            unpack_wind(hs[u], sx, sy, sz);
-            const uint32_t t = pack_wind(sx - wx, sy - wy, sz - wz);
+            unpack_wind(hs[u] & ~kHomeCell, sx, sy, sz);
+            const int32_t tx = sx - wx, ty = sy - wy, tz = sz - wz;
+            const uint32_t t = pack_wind(tx, ty, tz);
             if (t & kWindBig) L.flag = 1;
+            // 4-byte slot entries need every S = t + w_i within [-1, 1]^3: home
+            // atoms inside the cell (w_i = 0) and halo atoms with |t_k| <= 1
+            const bool ok = (hs[u] & kHomeCell) ? (tx | ty | tz) == 0
+                                                : ((unsigned)(tx + 1) <= 2u && (unsigned)(ty + 1) <= 2u &&
+                                                   (unsigned)(tz + 1) <= 2u);
+            if (!ok) L.row8 = 1;
             L.t[idx] = t;
             L.j[idx] = r[u].j;
             L.x[idx] = r[u].x;
@@ -509,34 +523,57 @@ __device__ __forceinline__ HomeRows brick_home(const Brick &B, const BrickLDS &L
     return R;
 }
 
-// LDS index of candidate k of a home cell (k < total).
-__device__ __forceinline__ int row_index(const HomeRows &R, int k)
+// LDS index of candidate k of a home cell (k < total): k plus the delta of
+// the last row whose exclusive prefix is <= k.  The nine (prefix, delta)
+// pairs are wave-uniform.
+struct RowTable {
+    int rex[9];   // exclusive candidate prefix of each row
+    int d0;       // LDS start of row 0
+    int inc[9];   // change of (LDS start - prefix) from row r - 1 to row r
+};
+
+__device__ __forceinline__ RowTable row_table(const HomeRows &R)
 {
-    int m = 0;
+    RowTable T;
+    int prev = 0;
 #pragma unroll
-    for (int step = 8; step >= 1; step >>= 1) {
-        const int e2 = __shfl(R.rex, m + step, kWave);
-        if (e2 <= k) m += step;
+    for (int r = 0; r < 9; ++r) {
+        T.rex[r] = __builtin_amdgcn_readlane(R.rex, r);
+        const int dl = __builtin_amdgcn_readlane(R.rstart, r) - T.rex[r];
+        T.inc[r] = dl - prev;
+        prev = dl;
     }
-    return __shfl(R.rstart, m, kWave) + (k - __shfl(R.rex, m, kWave));
+    T.d0 = T.inc[0];
+    return T;
+}
+
+__device__ __forceinline__ int row_index(const RowTable &T, int k)
+{
+    int d = T.d0;
+#pragma unroll
+    for (int r = 1; r < 9; ++r) d += k >= T.rex[r] ? T.inc[r] : 0;
+    return k + d;
 }
 
 // Append one ballot's hits to a home atom's slot row.  `row` and `rrow` are
 // the (wave-uniform) slot row and its R row; ca the atom's running count;
 // ent = j | pack(s - w_j) << 32 is the candidate's entry when w_i = 0.
-template <bool WANT_R, bool WIDE>
+template <bool NARROW, bool WZERO, bool WANT_R, bool WIDE>
 __device__ __forceinline__ void emit_row(void *__restrict__ row, double *__restrict__ rrow, uint32_t cap,
                                          Status *__restrict__ status, bool hit, uint64_t m, int32_t &ca, uint64_t ent,
-                                         bool wzero, int32_t wix, int32_t wiy, int32_t wiz, double Rx, double Ry,
-                                         double Rz)
+                                         int32_t wix, int32_t wiy, int32_t wiz, double Rx, double Ry, double Rz)
 {
-    const uint32_t r = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, (uint32_t)ca));
+    const uint32_t r =
+        __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) + (uint32_t)ca;
     if (hit && r < cap) {
-        if (WIDE) {
+        if (NARROW) {
+            // j | S code << 26 (w_i = 0 in a 4-byte brick)
+            reinterpret_cast<uint32_t *>(row)[r] = (uint32_t)ent;
+        } else if (WIDE) {
             int32_t tx, ty, tz;
             unpack_wind((uint32_t)(ent >> 32), tx, ty, tz);
             reinterpret_cast<int4 *>(row)[r] = make_int4((int32_t)(uint32_t)ent, tx + wix, ty + wiy, tz + wiz);
-        } else if (wzero) {
+        } else if (WZERO) {
             // S = s_loop + w_i - w_j = s_loop - w_j  (gpu_kernels.jl:88)
             reinterpret_cast<uint64_t *>(row)[r] = ent;
         } else {
@@ -554,6 +591,56 @@ __device__ __forceinline__ void emit_row(void *__restrict__ row, double *__restr
         }
     }
     ca += __popcll(m);
+}
+
+// One (home atom, chunk) test of the brick pass.  SHIFT: add the candidate's
+// cell shift C' S (left out when every lane's shift is zero and R is not
+// written: r^2 is then the same).  `dself` is the lane of the home atom's own
+// (j == i, s == 0) candidate in this chunk, or out of range.
+template <bool NARROW, bool SHIFT, bool WZERO, bool WANT_R, bool WIDE>
+__device__ __forceinline__ void test_chunk(double cx, double cy, double cz, double sx, double sy, double sz, double hx,
+                                           double hy, double hz, double rc2, int dself, void *__restrict__ row,
+                                           double *__restrict__ rrow, uint32_t cap, Status *__restrict__ status,
+                                           int32_t &ca, uint64_t ent, int32_t wix, int32_t wiy, int32_t wiz)
+{
+    // R = xj - xi + C' * S ; r^2 = dot(R, R)  (gpu_kernels.jl:89-92)
+    double Rx = cx - hx, Ry = cy - hy, Rz = cz - hz;
+    if (SHIFT) {
+        Rx = Rx + sx;
+        Ry = Ry + sy;
+        Rz = Rz + sz;
+    }
+    const double r2 = (Rx * Rx + Ry * Ry) + Rz * Rz;
+    const bool in = r2 < rc2;
+    // _is_self_interaction: j == i with no cell shift is the home atom's own
+    // slot in the home cell, lane dself of this chunk (if it is in range)
+    const uint64_t self = (unsigned)dself < (unsigned)kWave ? ~(1ull << dself) : ~0ull;
+    const uint64_t m = __builtin_amdgcn_ballot_w64(in) & self;
+    emit_row<NARROW, WZERO, WANT_R, WIDE>(row, rrow, cap, status, in && lane_id() != dself, m, ca, ent, wix, wiy, wiz,
+                                          Rx, Ry, Rz);
+}
+
+// All chunks of one home atom.
+template <bool NARROW, bool WZERO, bool WANT_R, bool WIDE>
+__device__ __forceinline__ void test_home_atom(int nch, const double *cx, const double *cy, const double *cz,
+                                               const double *sx, const double *sy, const double *sz, const bool *csh,
+                                               const uint64_t *ent, double hx, double hy, double hz, double rc2,
+                                               int ks, void *__restrict__ row, double *__restrict__ rrow, uint32_t cap,
+                                               Status *__restrict__ status, int32_t &ca, int32_t wix, int32_t wiy,
+                                               int32_t wiz)
+{
+#pragma unroll
+    for (int g = 0; g < kBrickChunks; ++g) {
+        if (g < nch) {
+            const int dself = ks - g * kWave;
+            if (WANT_R || csh[g])
+                test_chunk<NARROW, true, WZERO, WANT_R, WIDE>(cx[g], cy[g], cz[g], sx[g], sy[g], sz[g], hx, hy, hz, rc2,
+                                                              dself, row, rrow, cap, status, ca, ent[g], wix, wiy, wiz);
+            else
+                test_chunk<NARROW, false, WZERO, WANT_R, WIDE>(cx[g], cy[g], cz[g], sx[g], sy[g], sz[g], hx, hy, hz, rc2,
+                                                               dself, row, rrow, cap, status, ca, ent[g], wix, wiy, wiz);
+        }
+    }
 }
 
 // Brick pass of the count: home atoms stream through the wave as scalars
@@ -584,8 +671,10 @@ __global__ __launch_bounds__(kBrickThreads) void count_kernel(
     const double rc2 = sd.rc2;
     const int lane = lane_id();
     const int wv = sgpr((int)(threadIdx.x >> 6));
+    const bool narrow = !WIDE && sgpr(L.row8) == 0;   // the brick writes 4-byte slot entries
     for (int hc = wv; hc < B.nhome; hc += kBrickWaves) {
         const HomeRows R = brick_home(B, L, hc);
+        const RowTable T = row_table(R);
         const int nha = R.h1 - R.h0;
         for (int hg = 0; hg < nha; hg += kWave) {
             const int nh = sgpr(min(kWave, nha - hg));
@@ -593,7 +682,6 @@ __global__ __launch_bounds__(kBrickThreads) void count_kernel(
             const double hx_l = L.x[hl], hy_l = L.y[hl], hz_l = L.z[hl];
             const uint32_t ht_l = L.t[hl];   // pack(-w_i): a home cell's own shift is zero
             const int32_t hj_l = L.j[hl];
-            const int64_t p0 = (int64_t)R.pbase + hg;
             const int kself = R.self0 + hg;  // candidate index of home atom hg + a is kself + a
             int32_t cnt_l = 0;                // lane a: running count of home atom hg + a
             for (int k0 = 0; k0 < R.total; k0 += kBrickChunks * kWave) {
@@ -608,17 +696,23 @@ __global__ __launch_bounds__(kBrickThreads) void count_kernel(
                     if (g < nch) {
                         const int k = k0 + g * kWave + lane;
                         const bool v = k < R.total;
-                        const int li = row_index(R, v ? k : 0);
+                        const int li = row_index(T, v ? k : 0);
                         if (v) {
                             cx[g] = L.x[li];
                             cy[g] = L.y[li];
                             cz[g] = L.z[li];
-                            ent[g] = (uint64_t)(uint32_t)L.j[li] | ((uint64_t)L.t[li] << 32);
+                            const uint32_t t = L.t[li];
+                            // low word: j, or j | S code << 26 for 4-byte entries; the
+                            // 10-bit biased fields of t are the codes in their low 2 bits
+                            const uint32_t code = (t & 3u) | ((t >> 8) & 12u) | ((t >> 16) & 48u);
+                            const uint32_t lo = (uint32_t)L.j[li] | (narrow ? code << 26 : 0u);
+                            ent[g] = (uint64_t)lo | ((uint64_t)t << 32);
                         }
                     }
                 }
                 uint32_t wcur = 0xffffffffu;   // home winding the shifts were computed for
                 int32_t wix = 0, wiy = 0, wiz = 0;
+                bool csh[kBrickChunks];        // wave-uniform: some lane of the chunk has a nonzero shift
                 for (int a = 0; a < nh; ++a) {
                     const double hx = readlane_f64(hx_l, a), hy = readlane_f64(hy_l, a), hz = readlane_f64(hz_l, a);
                     const uint32_t ht = (uint32_t)__builtin_amdgcn_readlane((int)ht_l, a);
@@ -629,41 +723,41 @@ __global__ __launch_bounds__(kBrickThreads) void count_kernel(
 #pragma unroll
                         for (int g = 0; g < kBrickChunks; ++g) {
                             sx[g] = sy[g] = sz[g] = 0.0;
+                            csh[g] = false;
                             if (g < nch) {
                                 int32_t cx_, cy_, cz_;
                                 unpack_wind((uint32_t)(ent[g] >> 32), cx_, cy_, cz_);
-                                cell_shift(Csh, cx_ + wix, cy_ + wiy, cz_ + wiz, sx[g], sy[g], sz[g]);
+                                cx_ += wix; cy_ += wiy; cz_ += wiz;
+                                csh[g] = __builtin_amdgcn_ballot_w64((cx_ | cy_ | cz_) != 0) != 0;
+                                if (csh[g]) cell_shift(Csh, cx_, cy_, cz_, sx[g], sy[g], sz[g]);
                             }
                         }
                         wcur = ht;
                     }
-                    const bool wzero = ht == kPackZero;
+                    const bool wzero = ht == kPackZero;   // the home atom lies inside the cell
                     int32_t ca = __builtin_amdgcn_readlane(cnt_l, a);
-                    const int64_t srow = (p0 + a) * (int64_t)cap;
+                    // slot rows are indexed by original atom index, so the fill reads them in order
+                    const int64_t srow = (sd.atom_base + __builtin_amdgcn_readlane(hj_l, a)) * (int64_t)cap;
                     void *row = WIDE ? (void *)(reinterpret_cast<int4 *>(slots) + srow)
                                      : (void *)(reinterpret_cast<uint64_t *>(slots) + srow);
                     double *rrow = WANT_R ? slotsR + 3 * srow : nullptr;
                     const int ks = kself + a - k0;   // the self candidate's place in this pass
-#pragma unroll
-                    for (int g = 0; g < kBrickChunks; ++g) {
-                        if (g < nch) {
-                            // R = xj - xi + C' * S ; r^2 = dot(R, R)  (gpu_kernels.jl:89-92)
-                            const double Rx = (cx[g] - hx) + sx[g];
-                            const double Ry = (cy[g] - hy) + sy[g];
-                            const double Rz = (cz[g] - hz) + sz[g];
-                            const double r2 = (Rx * Rx + Ry * Ry) + Rz * Rz;
-                            // _is_self_interaction: j == i with no cell shift is the home
-                            // atom's own slot in the home cell, lane ks - 64 g of this chunk
-                            const bool hit = (r2 < rc2) & (lane != ks - g * kWave);
-                            const uint64_t m = __builtin_amdgcn_ballot_w64(hit);
-                            if (m) emit_row<WANT_R, WIDE>(row, rrow, (uint32_t)cap, status, hit, m, ca, ent[g], wzero,
-                                                          wix, wiy, wiz, Rx, Ry, Rz);
-                        }
-                    }
+                    if (narrow)   // every home atom of a 4-byte brick has w_i = 0
+                        test_home_atom<true, true, WANT_R, WIDE>(nch, cx, cy, cz, sx, sy, sz, csh, ent, hx, hy, hz, rc2,
+                                                                 ks, row, rrow, (uint32_t)cap, status, ca, wix, wiy,
+                                                                 wiz);
+                    else if (wzero)
+                        test_home_atom<false, true, WANT_R, WIDE>(nch, cx, cy, cz, sx, sy, sz, csh, ent, hx, hy, hz, rc2,
+                                                                  ks, row, rrow, (uint32_t)cap, status, ca, wix, wiy,
+                                                                  wiz);
+                    else
+                        test_home_atom<false, false, WANT_R, WIDE>(nch, cx, cy, cz, sx, sy, sz, csh, ent, hx, hy, hz,
+                                                                   rc2, ks, row, rrow, (uint32_t)cap, status, ca, wix,
+                                                                   wiy, wiz);
                     cnt_l = lane == a ? ca : cnt_l;
                 }
             }
-            if (lane < nh) pair_count[sd.atom_base + hj_l] = cnt_l;
+            if (lane < nh) pair_count[sd.atom_base + hj_l] = cnt_l | (narrow ? 0 : kRow8Bit);
         }
     }
 }
@@ -698,31 +792,75 @@ __global__ void reset_queue_kernel(Status *__restrict__ status)
 // Fill pass: atoms in their original order, one wave per 64 consecutive
 // atoms and one lane per pair, so the (i, j, S[, R]) stores of a wave cover
 // one contiguous range of the output arrays (fill_pairs_kernel!,
-// gpu_kernels.jl:159-180).
-template <bool WANT_R, bool WIDE, typename TI>
-__global__ __launch_bounds__(1024) void fill_kernel(
-    const int64_t *__restrict__ sys_atom_base, int nsys, int64_t nat, const int32_t *__restrict__ pair_count,
-    const int64_t *__restrict__ pair_off, const int32_t *__restrict__ inv_perm, const void *__restrict__ slots,
-    const double *__restrict__ slotsR, const Status *__restrict__ status, int32_t cap, int64_t pair_capacity,
-    TI *__restrict__ i_out, TI *__restrict__ j_out, TI *__restrict__ S_out, double *__restrict__ R_out)
+// gpu_kernels.jl:159-180).  Each lane takes kFillUnroll pairs per round (64
+// apart), issuing all their slot loads before any store.
+constexpr int kFillUnroll = 4;
+
+// Slot entries of the fill, read raw in one round and decoded in the next:
+// the loads carry no branch (8 bytes are read for every format) and their
+// results are not touched until the round after, so a wave keeps two rounds
+// of entry loads in flight.
+template <bool WIDE>
+struct RawEntry;
+template <>
+struct RawEntry<false> {
+    uint64_t e;
+    bool w8;
+};
+template <>
+struct RawEntry<true> {
+    int4 e;
+};
+
+template <bool WIDE>
+__device__ __forceinline__ RawEntry<WIDE> load_entry(const void *__restrict__ slots, int64_t rs, int r, bool row8)
 {
-    if (status->npairs > pair_capacity || status->max_count > cap || (!WIDE && status->wide)) return;
-    using E = SlotEntry<WIDE>;
+    RawEntry<WIDE> x;
+    if constexpr (WIDE) {
+        x.e = reinterpret_cast<const int4 *>(slots)[rs + r];
+    } else {
+        // 8-byte rows: entry rs + r; 4-byte rows: 4-byte entry 2 rs + r of
+        // the same row (the 8 bytes read stay inside the row: 4 r + 8 <= 8 cap)
+        const int64_t byte = row8 ? 8 * (rs + r) : 4 * (2 * rs + r);
+        __builtin_memcpy(&x.e, reinterpret_cast<const char *>(slots) + byte, 8);
+        x.w8 = row8;
+    }
+    return x;
+}
+
+template <bool WIDE>
+__device__ __forceinline__ void decode_entry(const RawEntry<WIDE> &x, int32_t &j, int32_t &Sx, int32_t &Sy,
+                                             int32_t &Sz)
+{
+    if constexpr (WIDE) {
+        j = x.e.x; Sx = x.e.y; Sy = x.e.z; Sz = x.e.w;
+    } else {
+        const uint32_t lo = (uint32_t)x.e, hi = (uint32_t)(x.e >> 32);
+        int32_t px, py, pz;
+        unpack_wind(hi, px, py, pz);
+        const uint32_t code = lo >> 26;
+        j = x.w8 ? (int32_t)lo : (int32_t)(lo & (uint32_t)(kNarrowMaxAtoms - 1));
+        Sx = x.w8 ? px : narrow_comp(code, 0);
+        Sy = x.w8 ? py : narrow_comp(code, 1);
+        Sz = x.w8 ? pz : narrow_comp(code, 2);
+    }
+}
+
+template <bool WANT_R, bool WIDE, typename TI>
+struct FillRound {
+    RawEntry<WIDE> e[kFillUnroll];
+    int64_t is[kFillUnroll], ridx[kFillUnroll];
+};
+
+// Locate and load the kFillUnroll pairs h0 + 64 u + lane of a wave's range.
+template <bool WANT_R, bool WIDE, typename TI>
+__device__ __forceinline__ void fill_load(FillRound<WANT_R, WIDE, TI> &F, int h0, int tot, int ex, int r8, int64_t row,
+                                          int64_t il, const void *__restrict__ slots)
+{
     const int lane = lane_id();
-    const int64_t a0 = ((int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(kWave - 1)));
-    if (a0 >= nat) return;
-    const int64_t a = a0 + lane;
-    const bool va = a < nat;
-    const int cnt = va ? pair_count[a] : 0;
-    const int64_t off = va ? pair_off[a] : 0;
-    const int64_t sbase = va ? (int64_t)inv_perm[a] * cap : 0;
-    int64_t il = a + 1;   // 1-based system-local index of atom a
-    if (nsys > 1 && va) il = a - sys_atom_base[find_sys(sys_atom_base, nsys, a)] + 1;
-    const int incl = wave_incl_scan(cnt);
-    const int tot = sgpr(__shfl(incl, kWave - 1, kWave));
-    const int ex = incl - cnt;
-    for (int h0 = 0; h0 < tot; h0 += kWave) {
-        const int L = h0 + lane;
+#pragma unroll
+    for (int u = 0; u < kFillUnroll; ++u) {
+        const int L = h0 + u * kWave + lane;
         int s = 0;   // atom of this pair: largest s with ex[s] <= L
 #pragma unroll
         for (int step = 32; step >= 1; step >>= 1) {
@@ -730,26 +868,94 @@ __global__ __launch_bounds__(1024) void fill_kernel(
             if (e2 <= L) s += step;
         }
         const int r = L - __shfl(ex, s, kWave);
-        const int64_t pos = (int64_t)shfl64((uint64_t)off, s) + r;
-        const int64_t sidx = (int64_t)shfl64((uint64_t)sbase, s) + r;
-        const int64_t is = (int64_t)shfl64((uint64_t)il, s);
+        const bool w8 = __shfl(r8, s, kWave) != 0;
+        const int64_t rs = (int64_t)shfl64((uint64_t)row, s);
+        F.is[u] = (int64_t)shfl64((uint64_t)il, s);
+        F.ridx[u] = rs + r;   // entry index in 8-byte (or 16-byte) units; R rows use it too
+        // lanes past the wave's pairs read entry 0 of the last atom's row (in
+        // bounds, never stored): no branch around the load
+        F.e[u] = load_entry<WIDE>(slots, rs, L < tot ? r : 0, w8);
+    }
+}
+
+template <bool WANT_R, bool WIDE, typename TI>
+__device__ __forceinline__ void fill_store(const FillRound<WANT_R, WIDE, TI> &F, int h0, int tot, int64_t off0,
+                                           const double *__restrict__ slotsR, TI *__restrict__ i_out,
+                                           TI *__restrict__ j_out, TI *__restrict__ S_out, double *__restrict__ R_out)
+{
+    const int lane = lane_id();
+#pragma unroll
+    for (int u = 0; u < kFillUnroll; ++u) {
+        const int L = h0 + u * kWave + lane;
+        int32_t j, Sx, Sy, Sz;
+        decode_entry<WIDE>(F.e[u], j, Sx, Sy, Sz);
         if (L < tot) {
-            int32_t j, Sx, Sy, Sz;
-            E::get(reinterpret_cast<const typename E::T *>(slots)[sidx], j, Sx, Sy, Sz);
-            i_out[pos] = (TI)is;
+            const int64_t pos = off0 + L;
+            i_out[pos] = (TI)F.is[u];
             j_out[pos] = (TI)(j + 1);
             TI *S = S_out + 3 * pos;
             S[0] = (TI)Sx;
             S[1] = (TI)Sy;
             S[2] = (TI)Sz;
             if (WANT_R) {
-                const double *Rs = slotsR + 3 * sidx;
+                const double *Rs = slotsR + 3 * F.ridx[u];
                 double *Rd = R_out + 3 * pos;
                 Rd[0] = Rs[0];
                 Rd[1] = Rs[1];
                 Rd[2] = Rs[2];
             }
         }
+    }
+}
+
+// AW atoms per wave (4 to 64): fewer atoms per wave make waves
+// shorter-lived, so the output ranges being written at any moment lie closer
+// together.
+template <bool WANT_R, bool WIDE, typename TI, int AW>
+__global__ __launch_bounds__(1024) void fill_kernel(
+    const int64_t *__restrict__ sys_atom_base, int nsys, int64_t nat, const int32_t *__restrict__ pair_count,
+    const int64_t *__restrict__ pair_off, const void *__restrict__ slots, const double *__restrict__ slotsR,
+    const Status *__restrict__ status, int32_t cap, int64_t pair_capacity, TI *__restrict__ i_out,
+    TI *__restrict__ j_out, TI *__restrict__ S_out, double *__restrict__ R_out)
+{
+    if (status->npairs > pair_capacity || status->max_count > cap || (!WIDE && status->wide)) return;
+    const int lane = lane_id();
+    const int64_t a0 = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * AW;
+    if (a0 >= nat) return;
+    const int64_t a = a0 + lane;
+    const bool va = lane < AW && a < nat;
+    const int32_t pc = va ? pair_count[a] : 0;
+    const int cnt = pc & kCountMask;
+    const int r8 = (pc & kRow8Bit) ? 1 : 0;
+    const int64_t off = va ? pair_off[a] : 0;
+    // slot row of atom a (rows are in original order): entry index (8-byte
+    // units) of its first entry; a 4-byte row is addressed in 4-byte units
+    // from the same start
+    const int64_t row = va ? a * (int64_t)cap : 0;
+    int64_t il = a + 1;   // 1-based system-local index of atom a
+    if (nsys > 1 && va) il = a - sys_atom_base[find_sys(sys_atom_base, nsys, a)] + 1;
+    const int incl = wave_incl_scan(cnt);
+    const int tot = sgpr(__shfl(incl, kWave - 1, kWave));
+    const int ex = incl - cnt;
+    const int64_t off0 = readlane64(off, 0);   // the wave's pairs are [off0, off0 + tot)
+    constexpr int kRound = kFillUnroll * kWave;
+    // software pipeline over two buffers: the next round's slot loads are in
+    // flight while this round's pairs are stored (no register copies between
+    // rounds, which would wait for the loads).  The empty asm keeps each
+    // round's loads ahead of the previous round's stores: one memory counter
+    // covers both, so a load issued after stores would make its wait drain
+    // them too.
+    FillRound<WANT_R, WIDE, TI> A, B;
+    if (tot > 0) fill_load(A, 0, tot, ex, r8, row, il, slots);
+    for (int h0 = 0; h0 < tot; h0 += 2 * kRound) {
+        const int h1 = h0 + kRound;
+        if (h1 < tot) fill_load(B, h1, tot, ex, r8, row, il, slots);
+        __asm__ volatile("" ::: "memory");
+        fill_store(A, h0, tot, off0, slotsR, i_out, j_out, S_out, R_out);
+        if (h1 >= tot) break;
+        if (h1 + kRound < tot) fill_load(A, h1 + kRound, tot, ex, r8, row, il, slots);
+        __asm__ volatile("" ::: "memory");
+        fill_store(B, h1, tot, off0, slotsR, i_out, j_out, S_out, R_out);
     }
 }
 
@@ -783,25 +989,36 @@ void launch_count(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st)
     }
 }
 
-template <bool R, bool W, typename TI>
+template <bool R, bool W, typename TI, int AW>
 static void fill_launch(const PipelineBufs &b, hipStream_t st)
 {
     const int bs = b.fill_block;
-    const unsigned nblk = (unsigned)((b.nat + bs - 1) / bs);
-    hipLaunchKernelGGL((fill_kernel<R, W, TI>), dim3(nblk), dim3(bs), 0, st, b.sys_atom_base, b.nsys, b.nat,
-                       b.pair_count, b.pair_off, b.inv_perm, b.slots, b.slotsR, b.status, b.slot_cap, b.pair_capacity,
+    const int64_t waves = (b.nat + AW - 1) / AW;
+    const unsigned nblk = (unsigned)((waves * kWave + bs - 1) / bs);
+    hipLaunchKernelGGL((fill_kernel<R, W, TI, AW>), dim3(nblk), dim3(bs), 0, st, b.sys_atom_base, b.nsys, b.nat,
+                       b.pair_count, b.pair_off, b.slots, b.slotsR, b.status, b.slot_cap, b.pair_capacity,
                        (TI *)b.i_out, (TI *)b.j_out, (TI *)b.S_out, b.R_out);
+}
+
+template <bool R, bool W, typename TI>
+static void fill_atoms(const PipelineBufs &b, hipStream_t st)
+{
+    if (b.fill_atoms == 4) fill_launch<R, W, TI, 4>(b, st);
+    else if (b.fill_atoms == 8) fill_launch<R, W, TI, 8>(b, st);
+    else if (b.fill_atoms == 16) fill_launch<R, W, TI, 16>(b, st);
+    else if (b.fill_atoms == 32) fill_launch<R, W, TI, 32>(b, st);
+    else fill_launch<R, W, TI, 64>(b, st);
 }
 
 template <typename TI>
 static void fill_dispatch(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st)
 {
     if (want_R) {
-        if (wide) fill_launch<true, true, TI>(b, st);
-        else fill_launch<true, false, TI>(b, st);
+        if (wide) fill_atoms<true, true, TI>(b, st);
+        else fill_atoms<true, false, TI>(b, st);
     } else {
-        if (wide) fill_launch<false, true, TI>(b, st);
-        else fill_launch<false, false, TI>(b, st);
+        if (wide) fill_atoms<false, true, TI>(b, st);
+        else fill_atoms<false, false, TI>(b, st);
     }
 }
 
