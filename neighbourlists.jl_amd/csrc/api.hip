@@ -539,6 +539,13 @@ int64_t estimate_pairs(const nlh_handle *h)
     return (int64_t)std::min<long double>(est, 4.0e11L) + 1024;
 }
 
+// Slot rows hold a multiple of 16 entries, so every row starts on a 128-byte
+// line (8-byte entries) and a row of up to 32 4-byte entries fills one line.
+int32_t round_slot_cap(double c)
+{
+    return (int32_t)((c + 15) / 16) * 16;
+}
+
 // Expected neighbours per atom, largest over the batch (a slot-size guess;
 // an underestimate costs one repeat of the count and fill passes).
 int32_t estimate_slot_cap(const nlh_handle *h)
@@ -552,7 +559,7 @@ int32_t estimate_slot_cap(const nlh_handle *h)
         mx = std::max(mx, nat / vol * 4.18879020478639 * g.cutoff * g.cutoff * g.cutoff);
     }
     const double c = std::min(mx * 1.5 + 16.0, 1.0e6);
-    return (int32_t)((c + 7) / 8) * 8;
+    return round_slot_cap(c);
 }
 
 void ensure_slots(nlh_handle *h, int32_t cap, bool want_R)
@@ -665,7 +672,7 @@ void materialize(nlh_handle *h, uint32_t flags)
         if (st.wide) h->wide = true;
         ensure_pair_outputs(h, std::max<int64_t>(st.npairs, 1), want_R);
         h->slots.release();
-        ensure_slots(h, (int32_t)((std::max(st.max_count, 1) + 7) / 8) * 8, want_R);
+        ensure_slots(h, round_slot_cap(std::max(st.max_count, 1)), want_R);
         check_pair_buffers(h, want_R);
         b = h->bufs();
         nlh::launch_count(b, want_R, h->wide, h->stream);

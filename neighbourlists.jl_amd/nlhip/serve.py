@@ -372,8 +372,8 @@ def serve_plain(requests: list[Request], *, realtime: bool = True, int_type=Int3
     n = C.c_int64()
     for r in order:
         if realtime:
-            while time.perf_counter() - t0 < r.arrival:
-                time.sleep(min(r.arrival - (time.perf_counter() - t0), 2e-4) if r.arrival > 0 else 0)
+            while (wait := r.arrival - (time.perf_counter() - t0)) > 0:
+                time.sleep(min(wait, 2e-4))
         start = time.perf_counter() - t0
         g = geometry(r.cell, r.cutoff, r.pbc, ti)
         X = np.ascontiguousarray(r.X, dtype=np.float64)
