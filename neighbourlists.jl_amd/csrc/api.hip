@@ -108,9 +108,15 @@ struct DevBuf {
 struct nlh_handle {
     int device = 0;
     hipStream_t stream = nullptr;
-    DevBuf X, sys, atom_base, cell_base, brick_base, key, slot, cell_count, cell_start, tmp, rec, wbig, perm, cell_id,
+    DevBuf X, tables, key, slot, cell_count, cell_start, tmp, rec, wbig, perm, cell_id,
         cell_offsets, pair_count, fbq, inv_perm, slots, slotsR, pair_off, partials, sys_pair_base, first, i, j, S, R, status, Xsorted, nb_j, nb_R,
         nb_S, nb_count;
+    // system tables, one device buffer: SysDev[nsys], atom bases int64[nsys + 1],
+    // cell bases int32[nsys + 1], brick bases int32[nsys + 1]
+    nlh::SysDev *sys_d = nullptr;
+    int64_t *atom_base_d = nullptr;
+    int32_t *cell_base_d = nullptr, *brick_base_d = nullptr;
+    std::vector<char> tables_host;        // bytes of the last upload
     nlh::Status *status_host = nullptr;   // pinned
     void *staging = nullptr;              // pinned upload staging for system tables
     size_t staging_cap = 0;
@@ -120,6 +126,7 @@ struct nlh_handle {
     int32_t M = 0;
     int ti_bytes = 4;
     bool built = false, materialized = false, have_R = false;
+    bool queue_clean = false;   // the count pass's brick queue is empty (status just cleared)
     int64_t npairs = 0;
     int64_t pair_capacity = 0;
     int32_t slot_cap = 0;       // neighbour slots per atom
@@ -151,10 +158,10 @@ struct nlh_handle {
         nlh::PipelineBufs b;
         std::memset(&b, 0, sizeof b);   // padding too: captured configurations are compared bytewise
         b.X = X_dev;
-        b.sys = sys.as<nlh::SysDev>();
-        b.sys_atom_base = atom_base.as<int64_t>();
-        b.sys_cell_base = cell_base.as<int32_t>();
-        b.sys_brick_base = brick_base.as<int32_t>();
+        b.sys = sys_d;
+        b.sys_atom_base = atom_base_d;
+        b.sys_cell_base = cell_base_d;
+        b.sys_brick_base = brick_base_d;
         b.nbricks = nbricks;
         b.nsys = nsys;
         b.nat = nat;
@@ -409,26 +416,35 @@ void assign_bricks(nlh_handle *h)
     h->fbq.ensure(h->stream, (size_t)std::max<int64_t>(bricks, 1) * 4);
 }
 
-// Upload the system tables (sys, atom/cell/brick bases) through pinned
-// staging.  Waits for earlier work on the stream first: staging is reused.
+// Upload the system tables (sys, atom/cell/brick bases) as one copy through
+// pinned staging, unless the device already holds exactly these bytes (a
+// rebuild of the same systems).  Waits for earlier work on the stream before
+// reusing the staging buffer.
 void upload_tables(nlh_handle *h)
 {
     const int nsys = (int)h->sys_host.size();
     const size_t sys_bytes = nsys * sizeof(nlh::SysDev);
     const size_t ab_bytes = (nsys + 1) * sizeof(int64_t);
     const size_t cb_bytes = (nsys + 1) * sizeof(int32_t);
+    const size_t o_ab = (sys_bytes + 15) / 16 * 16, o_cb = o_ab + (ab_bytes + 15) / 16 * 16,
+                 o_bb = o_cb + (cb_bytes + 15) / 16 * 16, total = o_bb + cb_bytes;
+    std::vector<char> img(total, 0);
+    std::memcpy(img.data(), h->sys_host.data(), sys_bytes);
+    std::memcpy(img.data() + o_ab, h->atom_base_host.data(), ab_bytes);
+    std::memcpy(img.data() + o_cb, h->cell_base_host.data(), cb_bytes);
+    std::memcpy(img.data() + o_bb, h->brick_base_host.data(), cb_bytes);
+    const bool moved = h->tables.ensure(h->stream, total);
+    char *base = static_cast<char *>(h->tables.p);
+    h->sys_d = reinterpret_cast<nlh::SysDev *>(base);
+    h->atom_base_d = reinterpret_cast<int64_t *>(base + o_ab);
+    h->cell_base_d = reinterpret_cast<int32_t *>(base + o_cb);
+    h->brick_base_d = reinterpret_cast<int32_t *>(base + o_bb);
+    if (!moved && img == h->tables_host) return;
     HIPC(hipStreamSynchronize(h->stream));
-    ensure_staging(h, sys_bytes + ab_bytes + 2 * cb_bytes + 64);
-    char *st = static_cast<char *>(h->staging);
-    std::memcpy(st, h->sys_host.data(), sys_bytes);
-    std::memcpy(st + sys_bytes, h->atom_base_host.data(), ab_bytes);
-    std::memcpy(st + sys_bytes + ab_bytes, h->cell_base_host.data(), cb_bytes);
-    std::memcpy(st + sys_bytes + ab_bytes + cb_bytes, h->brick_base_host.data(), cb_bytes);
-    HIPC(hipMemcpyAsync(h->sys.p, st, sys_bytes, hipMemcpyHostToDevice, h->stream));
-    HIPC(hipMemcpyAsync(h->atom_base.p, st + sys_bytes, ab_bytes, hipMemcpyHostToDevice, h->stream));
-    HIPC(hipMemcpyAsync(h->cell_base.p, st + sys_bytes + ab_bytes, cb_bytes, hipMemcpyHostToDevice, h->stream));
-    HIPC(hipMemcpyAsync(h->brick_base.p, st + sys_bytes + ab_bytes + cb_bytes, cb_bytes, hipMemcpyHostToDevice,
-                        h->stream));
+    ensure_staging(h, total);
+    std::memcpy(h->staging, img.data(), total);
+    HIPC(hipMemcpyAsync(h->tables.p, h->staging, total, hipMemcpyHostToDevice, h->stream));
+    h->tables_host.swap(img);
 }
 
 // Set up the system tables of a batch (a single build is nsys == 1) and
@@ -468,10 +484,6 @@ void setup_batch(nlh_handle *h, int32_t nsys, const int64_t *atom_offsets, const
 
     const size_t n1 = (size_t)std::max<int64_t>(nat, 1);
     const size_t M1 = (size_t)cells + 1;
-    h->sys.ensure(h->stream, nsys * sizeof(nlh::SysDev));
-    h->atom_base.ensure(h->stream, (nsys + 1) * sizeof(int64_t));
-    h->cell_base.ensure(h->stream, (nsys + 1) * sizeof(int32_t));
-    h->brick_base.ensure(h->stream, (nsys + 1) * sizeof(int32_t));
     h->key.ensure(h->stream, n1 * 4);
     h->slot.ensure(h->stream, n1 * 4);
     h->cell_count.ensure(h->stream, M1 * 4, /*zero=*/true);
@@ -504,6 +516,15 @@ void upload_positions(nlh_handle *h, const double *X, uint32_t flags)
 void reset_status(nlh_handle *h)
 {
     nlh::launch_clear_status(h->status.as<nlh::Status>(), h->stream);
+    h->queue_clean = true;
+}
+
+// Every count launch starts from an empty fallback-brick queue.
+void count_pass(nlh_handle *h, const nlh::PipelineBufs &b, bool want_R)
+{
+    if (!h->queue_clean) nlh::launch_reset_queue(b.status, h->stream);
+    nlh::launch_count(b, want_R, h->wide, h->stream);
+    h->queue_clean = false;
 }
 
 void mark(nlh_handle *h, int k)
@@ -655,7 +676,7 @@ void materialize(nlh_handle *h, uint32_t flags)
     check_pair_buffers(h, want_R);
     nlh::PipelineBufs b = h->bufs();
     mark(h, 1);
-    nlh::launch_count(b, want_R, h->wide, h->stream);
+    count_pass(h, b, want_R);
     mark(h, 2);
     nlh::launch_scan_pairs(b, h->stream);
     mark(h, 3);
@@ -675,7 +696,7 @@ void materialize(nlh_handle *h, uint32_t flags)
         ensure_slots(h, round_slot_cap(std::max(st.max_count, 1)), want_R);
         check_pair_buffers(h, want_R);
         b = h->bufs();
-        nlh::launch_count(b, want_R, h->wide, h->stream);
+        count_pass(h, b, want_R);
         nlh::launch_fill(b, want_R, h->wide, h->stream);
         HIPC(hipGetLastError());
         finish_status(h);
@@ -720,7 +741,7 @@ void capture_graph(nlh_handle *h, bool want_R)
         // kernel launches only between begin and end (nothing that takes the lock)
         std::unique_lock<std::shared_mutex> lk(g_capture_mutex);
         HIPC(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-        nlh::launch_clear_status(b.status, h->stream);
+        nlh::launch_clear_status(b.status, h->stream);   // also empties the brick queue
         nlh::launch_build(b, h->stream);
         nlh::launch_count(b, want_R, h->wide, h->stream);
         nlh::launch_scan_pairs(b, h->stream);
@@ -751,6 +772,7 @@ bool rebuild_graph(nlh_handle *h, bool want_R)
     if (!h->graph_exec || h->graph_want_R != want_R || std::memcmp(&b, &h->graph_bufs, sizeof b) != 0)
         capture_graph(h, want_R);
     HIPC(hipGraphLaunch(h->graph_exec, h->stream));
+    h->queue_clean = false;
     ++h->graph_launches;
     h->built = true;
     h->materialized = false;
@@ -907,7 +929,7 @@ int nlh_destroy(nlh_handle *h)
             for (hipEvent_t &e : h->ev)
                 if (e) (void)hipEventDestroy(e);
         });
-        DevBuf *bufs[] = {&h->X,        &h->sys,       &h->atom_base,  &h->cell_base, &h->brick_base,    &h->key,    &h->slot,
+        DevBuf *bufs[] = {&h->X,        &h->tables,    &h->key,    &h->slot,
                           &h->cell_count, &h->cell_start, &h->tmp,      &h->rec,          &h->wbig,   &h->perm,
                           &h->cell_id,  &h->cell_offsets, &h->pair_count, &h->fbq, &h->inv_perm, &h->slots, &h->slotsR, &h->pair_off,   &h->partials,
                           &h->sys_pair_base, &h->first, &h->i,          &h->j,            &h->S,      &h->R,

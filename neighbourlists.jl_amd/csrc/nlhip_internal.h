@@ -17,7 +17,7 @@ constexpr int kWave = 64;
 constexpr int kPairsBlock = 256;          // 4 waves, one home cell per wave
 constexpr int kWavesPerPairsBlock = kPairsBlock / kWave;
 constexpr int kScanThreads = 256;
-constexpr int kScanItems = 16;            // elements per thread in a scan tile
+constexpr int kScanItems = 8;             // elements per thread in a scan tile
 constexpr int kScanTile = kScanThreads * kScanItems;
 constexpr int kBrickAtomBudget = 1536;    // expected halo atoms per brick (LDS holds 2048)
 
@@ -205,6 +205,7 @@ struct PipelineBufs {
 void launch_clear_status(Status *status, hipStream_t st);
 void launch_build(const PipelineBufs &b, hipStream_t st);
 void launch_count(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st);
+void launch_reset_queue(Status *status, hipStream_t st);
 void launch_scan_pairs(const PipelineBufs &b, hipStream_t st);
 void launch_fill(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st);
 void launch_sorted_positions(const PipelineBufs &b, double *Xs, hipStream_t st);

@@ -967,14 +967,20 @@ __global__ __launch_bounds__(1024) void fill_kernel(
 template <bool R, bool W>
 static void count_launch(const PipelineBufs &b, hipStream_t st)
 {
-    hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, st, b.status);
     hipLaunchKernelGGL((count_kernel<R, W>), dim3((unsigned)b.nbricks), dim3(kBrickThreads), 0, st, b.sys,
                        b.sys_cell_base, b.sys_brick_base, b.nsys, b.M, b.nat, b.cell_start, b.rec, b.wbig,
                        b.pair_count, b.slots, b.slotsR, b.slot_cap, b.status, b.fb_queue);
-    const unsigned nfb = (unsigned)std::min<int64_t>(b.nbricks, 1024);
+    const unsigned nfb = (unsigned)std::min<int64_t>(b.nbricks, 512);
     hipLaunchKernelGGL((count_fallback_kernel<R, W>), dim3(nfb), dim3(kFallbackThreads), 0, st, b.sys,
                        b.sys_cell_base, b.sys_brick_base, b.nsys, b.M, b.nat, b.cell_start, b.rec, b.wbig,
                        b.pair_count, b.slots, b.slotsR, b.slot_cap, b.status, (const int32_t *)b.fb_queue);
+}
+
+// The fallback queue is cleared with the rest of the status at the start of
+// a build; a repeated count (after the buffers grew) clears it here first.
+void launch_reset_queue(Status *status, hipStream_t st)
+{
+    hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, st, status);
 }
 
 void launch_count(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st)

@@ -201,3 +201,19 @@ def test_golden_fixture(path):
     cl = nlhip.build_cell_list(X, rc, C, pbc)
     for k in ("perm", "cell_id", "cell_offsets"):
         assert np.array_equal(getattr(cl, k), f[k]), k
+
+
+def test_materialize_twice():
+    """Repeated materialisation of one cell list (with and without R) gives
+    the same pairs; the count pass's fallback-brick queue starts empty each
+    time (cells smaller than the cutoff send every brick to that queue)."""
+    X = 5.0 * workloads.uniform(19, 300).reshape(100, 3)
+    for rc in (2.0, 1.2):
+        cl = nlhip.build_cell_list(X, rc, np.eye(3) * 5.0, (True, True, True))
+        a = nlhip.materialize_pairlist(cl)
+        b = nlhip.materialize_pairlist(cl, want_R=True)
+        c = nlhip.materialize_pairlist(cl)
+        ref = oracle.neighbour_list(X, rc, np.eye(3) * 5.0, (True, True, True), want_R=True)
+        for nl in (a, b, c):
+            assert np.array_equal(nl.i, ref.i) and np.array_equal(nl.j, ref.j) and np.array_equal(nl.S, ref.S)
+        assert np.array_equal(np.asarray(b.R).view(np.int64), ref.R.view(np.int64))
