@@ -139,6 +139,8 @@ def _batched(nsys: int, steps: int):
     Xd = _lib.DeviceArray(X)
     bb = BatchBuilder()
     bb.run(systems, X_device=Xd, copy=False)   # warm-up (buffers, capacities)
+    tuned = bb.autotune(X, X_device=Xd, reps=1)   # launch configuration for this batch (untimed)
+    bb.rebuild(X, X_device=Xd, copy=False)     # capture the pipeline with it
     _sync()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -168,6 +170,8 @@ def _batched(nsys: int, steps: int):
     tp = time.perf_counter() - t0
     assert Pp == P, (Pp, P)
     return {"nsys": nsys, "atoms": int(offs[-1]), "pairs": int(P), "batched_ms": round(tb * 1e3, 3),
+            "tuning": {"brick_atoms": tuned.brick_atoms, "fill_block": tuned.fill_block,
+                       "fill_atoms": tuned.fill_atoms, "count_block": tuned.count_block},
             "batched_systems_per_s": round(nsys / tb, 1), "plain_ms": round(tp * 1e3, 3),
             "plain_systems_per_s": round(nsys / tp, 1), "speedup": round(tp / tb, 2)}
 

@@ -151,7 +151,11 @@ struct nlh_handle {
     nlh::PipelineBufs graph_bufs;
     int64_t graph_launches = 0;
     // launch configuration (nlh_set_tuning / nlh_autotune)
-    nlh_tuning tune = {nlh::kBrickAtomBudget, 256, 8};
+    nlh_tuning tune = {nlh::kBrickAtomBudget, 256, 8, 0};
+
+    // count-pass workgroup: 8 waves per brick for one system, 4 for a batch
+    // (whose systems are mostly one small brick each), unless tuned
+    int count_block() const { return tune.count_block ? tune.count_block : (nsys > 1 ? 256 : 512); }
 
     nlh::PipelineBufs bufs() const
     {
@@ -195,6 +199,7 @@ struct nlh_handle {
         b.ti_bytes = ti_bytes;
         b.fill_block = tune.fill_block;
         b.fill_atoms = tune.fill_atoms;
+        b.count_block = count_block();
         return b;
     }
 };
@@ -365,7 +370,8 @@ void drop_graph(nlh_handle *h)
 // Pair-search brick of a system: the largest of a fixed ladder of shapes
 // whose halo is expected to fit the LDS budget at this system's mean atom
 // density per cell (clipped to the cell grid).
-void choose_brick(const nlh_geometry &g, int64_t nat, int32_t budget, int32_t out[3], int64_t min_bricks = 0)
+void choose_brick(const nlh_geometry &g, int64_t nat, int32_t budget, int32_t halo_max, int32_t out[3],
+                  int64_t min_bricks = 0)
 {
     static const int32_t ladder[][3] = {{4, 4, 4}, {4, 4, 2}, {4, 2, 2}, {2, 2, 2}, {2, 2, 1}, {2, 1, 1}, {1, 1, 1}};
     const double per_cell = g.ncells_total > 0 ? (double)nat / (double)g.ncells_total : 0.0;
@@ -379,7 +385,7 @@ void choose_brick(const nlh_geometry &g, int64_t nat, int32_t budget, int32_t ou
             halo *= (double)(d[k] + 2);
             nb *= (g.ncells[k] + d[k] - 1) / d[k];
         }
-        if (halo * per_cell * 1.25 <= (double)budget && halo <= 512.0) {
+        if (halo * per_cell * 1.25 <= (double)budget && halo <= (double)halo_max) {
             // the first shape that fits; a smaller one while the grid is too
             // coarse to fill the device, down to one home cell per wave
             std::memcpy(out, d, sizeof d);
@@ -404,7 +410,10 @@ void assign_bricks(nlh_handle *h)
         nlh::SysDev &sd = h->sys_host[s];
         // a single system splits into at least two rounds of bricks over the
         // device (2 bricks per CU fit its LDS)
-        choose_brick(h->geoms[s], sd.nat, h->tune.brick_atoms, sd.brick, nsys == 1 ? kMinBricks : 0);
+        // the LDS of a BT-thread brick holds 4 BT halo atoms and BT halo cells
+        const int bt = h->count_block();
+        choose_brick(h->geoms[s], sd.nat, std::min(h->tune.brick_atoms, 3 * bt), bt, sd.brick,
+                     nsys == 1 ? kMinBricks : 0);
         sd.brick_base = (int32_t)bricks;
         h->brick_base_host[s] = (int32_t)bricks;
         bricks += (int64_t)((sd.n[0] + sd.brick[0] - 1) / sd.brick[0]) * ((sd.n[1] + sd.brick[1] - 1) / sd.brick[1]) *
@@ -476,8 +485,8 @@ void setup_batch(nlh_handle *h, int32_t nsys, const int64_t *atom_offsets, const
     if (nat > INT32_MAX - 1) throw Error(NLH_EOVERFLOW, "more than 2^31-1 atoms in one build");
     h->atom_base_host[nsys] = nat;
     h->cell_base_host[nsys] = (int32_t)cells;
+    h->nsys = nsys;   // before the bricks: the brick workgroup size depends on it
     assign_bricks(h);
-    h->nsys = nsys;
     h->nat = nat;
     h->M = (int32_t)cells;
     h->ti_bytes = ti;
@@ -801,12 +810,14 @@ void validate_tuning(const nlh_tuning &t)
         throw Error(NLH_EINVAL, "fill_block must be a multiple of 64 in [64, 1024]");
     if (t.fill_atoms != 4 && t.fill_atoms != 8 && t.fill_atoms != 16 && t.fill_atoms != 32 && t.fill_atoms != 64)
         throw Error(NLH_EINVAL, "fill_atoms must be 4, 8, 16, 32 or 64");
+    if (t.count_block != 0 && t.count_block != 256 && t.count_block != 512)
+        throw Error(NLH_EINVAL, "count_block must be 0 (automatic), 256 or 512");
 }
 
 void apply_tuning(nlh_handle *h, const nlh_tuning &t)
 {
     validate_tuning(t);
-    const bool bricks = t.brick_atoms != h->tune.brick_atoms;
+    const bool bricks = t.brick_atoms != h->tune.brick_atoms || t.count_block != h->tune.count_block;
     h->tune = t;
     if (bricks && !h->sys_host.empty()) {
         assign_bricks(h);
@@ -1241,13 +1252,16 @@ int nlh_autotune(nlh_handle *h, const double *X_batch, uint32_t flags, int32_t r
         float win_ms = 0.0f;
         bool have = false;
         try {
-            static const int32_t bricks[] = {512, 1024, 1536};
+            static const int32_t bricks[] = {384, 768, 1536};
             static const int32_t fills[] = {128, 256, 512};
             static const int32_t atoms[] = {4, 8, 16};
+            static const int32_t blocks[] = {256, 512};
+            for (int32_t cb : blocks)
             for (int32_t ba : bricks)
                 for (int32_t fb : fills)
                     for (int32_t fa : atoms) {
-                    apply_tuning(h, nlh_tuning{ba, fb, fa});
+                    if (cb == 256 && ba > 768) continue;   // above what a 256-thread brick holds
+                    apply_tuning(h, nlh_tuning{ba, fb, fa, cb});
                     rebuild_eager(h, flags & ~NLH_NO_SYNC);   // warm-up (and buffer growth)
                     float tot = 0.0f;
                     for (int r = 0; r < reps; ++r) {

@@ -199,7 +199,8 @@ struct PipelineBufs {
     Status *status;
     int ti_bytes;
     int fill_block;          // threads per fill workgroup (multiple of 64)
-    int fill_atoms;          // atoms per fill wave (16, 32 or 64)
+    int fill_atoms;          // atoms per fill wave (4 to 64)
+    int count_block;         // threads per count-pass brick workgroup (256 or 512)
 };
 
 void launch_clear_status(Status *status, hipStream_t st);

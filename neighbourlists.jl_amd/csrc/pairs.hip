@@ -324,17 +324,20 @@ __device__ __forceinline__ void count_cell(const PairsArgs &A, const SysDev &sd,
     }
 }
 
-constexpr int kBrickThreads = 512;                       // 8 waves share one brick
-constexpr int kBrickWaves = kBrickThreads / kWave;
-constexpr int kBrickCap = 2048;                          // halo atoms held in LDS
-constexpr int kHaloMax = 512;                            // halo cells
-constexpr int kHaloPerThread = kHaloMax / kBrickThreads;
-constexpr int kBrickUnroll = kBrickCap / kBrickThreads;
+// Brick workgroup shapes: BT threads (8 waves for large systems, 4 for
+// batches of small ones), 4 BT halo atoms and BT halo cells in LDS.
+template <int BT>
+struct BrickShape {
+    static constexpr int threads = BT;
+    static constexpr int waves = BT / kWave;
+    static constexpr int cap = 4 * BT;    // halo atoms held in LDS
+    static constexpr int halo = BT;       // halo cells, one per thread
+    static constexpr int unroll = cap / BT;
+};
 constexpr int kFallbackThreads = 256;
 constexpr int kFallbackWaves = kFallbackThreads / kWave;
 constexpr uint32_t kPackZero = (512u) | (512u << 10) | (512u << 20);   // pack_wind(0, 0, 0)
 constexpr int kBrickChunks = 3;   // candidate chunks a lane holds in the brick pass (192 candidates)
-static_assert(kHaloPerThread >= 1 && kHaloPerThread * kBrickThreads == kHaloMax, "halo cells per thread");
 
 struct Brick {
     const SysDev *sd;
@@ -346,7 +349,7 @@ struct Brick {
 };
 
 __device__ __forceinline__ void brick_locate(const PairsArgs &A, const int32_t *__restrict__ sys_brick_base, int b,
-                                             Brick &B)
+                                             int halo_max, Brick &B)
 {
     const int s = A.nsys == 1 ? 0 : find_sys(sys_brick_base, A.nsys, (int32_t)b);
     const SysDev &sd = A.sys[s];
@@ -362,7 +365,7 @@ __device__ __forceinline__ void brick_locate(const PairsArgs &A, const int32_t *
     }
     B.nhalo = B.hd[0] * B.hd[1] * B.hd[2];
     B.nhome = B.nh[0] * B.nh[1] * B.nh[2];
-    B.fast = sd.nxyz[0] == 1 && sd.nxyz[1] == 1 && sd.nxyz[2] == 1 && B.nhalo <= kHaloMax;
+    B.fast = sd.nxyz[0] == 1 && sd.nxyz[1] == 1 && sd.nxyz[2] == 1 && B.nhalo <= halo_max;
 }
 
 __device__ __forceinline__ int brick_cell_id(const Brick &B, int hc)
@@ -372,13 +375,15 @@ __device__ __forceinline__ int brick_cell_id(const Brick &B, int hc)
     return sd.cell_base + (B.b0[0] - 1 + ix) + (B.b0[1] - 1 + iy) * sd.n[0] + (B.b0[2] - 1 + iz) * sd.n[0] * sd.n[1];
 }
 
+template <int BT>
 struct BrickLDS {
-    double x[kBrickCap], y[kBrickCap], z[kBrickCap];
-    int32_t j[kBrickCap];
-    uint32_t t[kBrickCap];          // pack(s_halo - w_j); home cells have s_halo = 0
-    int32_t hoff[kHaloMax + 1];     // LDS offset of each halo cell's atoms
-    int32_t hstart[kHaloMax];       // first sorted index of each halo cell
-    uint32_t hshift[kHaloMax];      // packed cell shift of each halo cell | kHomeCell
+    using Sh = BrickShape<BT>;
+    double x[Sh::cap], y[Sh::cap], z[Sh::cap];
+    int32_t j[Sh::cap];
+    uint32_t t[Sh::cap];            // pack(s_halo - w_j); home cells have s_halo = 0
+    int32_t hoff[Sh::halo + 1];     // LDS offset of each halo cell's atoms
+    int32_t hstart[Sh::halo];       // first sorted index of each halo cell
+    uint32_t hshift[Sh::halo];      // packed cell shift of each halo cell | kHomeCell
     int32_t total, flag;
     int32_t row8;                   // some shift needs 8-byte slot entries
 };
@@ -387,18 +392,20 @@ constexpr uint32_t kHomeCell = 1u << 30;   // halo-table mark of the brick's own
 
 // Stage the brick and its halo in LDS.  Returns false (for every thread of
 // the block) when the brick must take the global-memory path.
-__device__ bool brick_load(const PairsArgs &A, const Brick &B, BrickLDS &L)
+template <int BT>
+__device__ bool brick_load(const PairsArgs &A, const Brick &B, BrickLDS<BT> &L)
 {
+    using Sh = BrickShape<BT>;
     const SysDev &sd = *B.sd;
     const int tid = threadIdx.x;
     const bool p[3] = {(sd.pbc & 1) != 0, (sd.pbc & 2) != 0, (sd.pbc & 4) != 0};
-    // halo cell table: thread tid owns halo cells kHaloPerThread * tid + u
-    int cnt[kHaloPerThread];
+    // halo cell table: thread tid owns halo cell tid
+    int cnt[1];
     int sum = 0;
 #pragma unroll
-    for (int u = 0; u < kHaloPerThread; ++u) {
+    for (int u = 0; u < 1; ++u) {
         cnt[u] = 0;
-        const int h = kHaloPerThread * tid + u;
+        const int h = tid + u;
         if (h < B.nhalo) {
             const int ux = h % B.hd[0], uy = (h / B.hd[0]) % B.hd[1], uz = h / (B.hd[0] * B.hd[1]);
             const int a[3] = {B.b0[0] - 1 + ux, B.b0[1] - 1 + uy, B.b0[2] - 1 + uz};
@@ -421,10 +428,10 @@ __device__ bool brick_load(const PairsArgs &A, const Brick &B, BrickLDS &L)
         sum += cnt[u];
     }
     int64_t tot;
-    int run = (int)block_excl_scan_n<kBrickThreads>(sum, tot);
+    int run = (int)block_excl_scan_n<BT>(sum, tot);
 #pragma unroll
-    for (int u = 0; u < kHaloPerThread; ++u) {
-        const int h = kHaloPerThread * tid + u;
+    for (int u = 0; u < 1; ++u) {
+        const int h = tid + u;
         if (h < B.nhalo) L.hoff[h] = run;
         run += cnt[u];
     }
@@ -436,13 +443,13 @@ __device__ bool brick_load(const PairsArgs &A, const Brick &B, BrickLDS &L)
     }
     __syncthreads();
     const int total = L.total;
-    if (total > kBrickCap) return false;
+    if (total > Sh::cap) return false;
     // halo atoms: all loads of a thread in flight together
-    int pg[kBrickUnroll];
-    uint32_t hs[kBrickUnroll];
+    int pg[Sh::unroll];
+    uint32_t hs[Sh::unroll];
 #pragma unroll
-    for (int u = 0; u < kBrickUnroll; ++u) {
-        const int idx = tid + u * kBrickThreads;
+    for (int u = 0; u < Sh::unroll; ++u) {
+        const int idx = tid + u * BT;
         pg[u] = 0;
         hs[u] = kPackZero;
         if (idx < total) {
@@ -456,13 +463,13 @@ __device__ bool brick_load(const PairsArgs &A, const Brick &B, BrickLDS &L)
             hs[u] = L.hshift[lo];
         }
     }
-    Rec r[kBrickUnroll];
+    Rec r[Sh::unroll];
 #pragma unroll
-    for (int u = 0; u < kBrickUnroll; ++u)
-        if (tid + u * kBrickThreads < total) r[u] = A.rec[pg[u]];
+    for (int u = 0; u < Sh::unroll; ++u)
+        if (tid + u * BT < total) r[u] = A.rec[pg[u]];
 #pragma unroll
-    for (int u = 0; u < kBrickUnroll; ++u) {
-        const int idx = tid + u * kBrickThreads;
+    for (int u = 0; u < Sh::unroll; ++u) {
+        const int idx = tid + u * BT;
         if (idx < total) {
             int32_t wx, wy, wz, sx, sy, sz;
             rec_winding(r[u], A.wbig, pg[u], wx, wy, wz);
@@ -498,7 +505,8 @@ struct HomeRows {
     int self0;       // candidate index of home atom 0 (uniform)
 };
 
-__device__ __forceinline__ HomeRows brick_home(const Brick &B, const BrickLDS &L, int hc)
+template <int BT>
+__device__ __forceinline__ HomeRows brick_home(const Brick &B, const BrickLDS<BT> &L, int hc)
 {
     HomeRows R;
     const int ix = hc % B.nh[0], iy = (hc / B.nh[0]) % B.nh[1], iz = hc / (B.nh[0] * B.nh[1]);
@@ -648,19 +656,19 @@ __device__ __forceinline__ void test_home_atom(int nch, const double *cx, const 
 // registers, chunk by chunk; each (home atom, chunk) is one FP64 distance
 // test and one ballot.  Bricks that cannot take this path are queued for
 // count_fallback_kernel.
-template <bool WANT_R, bool WIDE>
-__global__ __launch_bounds__(kBrickThreads) void count_kernel(
+template <bool WANT_R, bool WIDE, int BT>
+__global__ __launch_bounds__(BT) void count_kernel(
     const SysDev *__restrict__ sys, const int32_t *__restrict__ sys_cell_base, const int32_t *__restrict__ sys_brick_base,
     int nsys, int32_t M, int64_t nat, const int32_t *__restrict__ cell_start, const Rec *__restrict__ rec,
     const int32_t *__restrict__ wbig, int32_t *__restrict__ pair_count, void *__restrict__ slots,
     double *__restrict__ slotsR, int32_t cap, Status *__restrict__ status, int32_t *__restrict__ fb_queue)
 {
-    __shared__ BrickLDS L;
+    __shared__ BrickLDS<BT> L;
     __shared__ double Csh[9];
     const PairsArgs A{sys, sys_cell_base, nsys, M, nat, cell_start, rec, wbig};
     const int b = (int)xcd_remap(blockIdx.x, gridDim.x);
     Brick B;
-    brick_locate(A, sys_brick_base, b, B);
+    brick_locate(A, sys_brick_base, b, BrickShape<BT>::halo, B);
     if (threadIdx.x < 9) Csh[threadIdx.x] = B.sd->C[threadIdx.x];   // visible after brick_load's barrier
     const bool lds = B.fast && brick_load(A, B, L);
     if (!lds) {
@@ -672,7 +680,7 @@ __global__ __launch_bounds__(kBrickThreads) void count_kernel(
     const int lane = lane_id();
     const int wv = sgpr((int)(threadIdx.x >> 6));
     const bool narrow = !WIDE && sgpr(L.row8) == 0;   // the brick writes 4-byte slot entries
-    for (int hc = wv; hc < B.nhome; hc += kBrickWaves) {
+    for (int hc = wv; hc < B.nhome; hc += BrickShape<BT>::waves) {
         const HomeRows R = brick_home(B, L, hc);
         const RowTable T = row_table(R);
         const int nha = R.h1 - R.h0;
@@ -778,7 +786,7 @@ __global__ __launch_bounds__(kFallbackThreads) void count_fallback_kernel(
     const int wv = sgpr((int)(threadIdx.x >> 6));
     for (int q = blockIdx.x; q < nq; q += gridDim.x) {
         Brick B;
-        brick_locate(A, sys_brick_base, fb_queue[q], B);
+        brick_locate(A, sys_brick_base, fb_queue[q], 0, B);
         for (int hc = wv; hc < B.nhome; hc += kFallbackWaves)
             count_cell<WANT_R, WIDE>(A, *B.sd, brick_cell_id(B, hc), pair_count, O);
     }
@@ -967,9 +975,14 @@ __global__ __launch_bounds__(1024) void fill_kernel(
 template <bool R, bool W>
 static void count_launch(const PipelineBufs &b, hipStream_t st)
 {
-    hipLaunchKernelGGL((count_kernel<R, W>), dim3((unsigned)b.nbricks), dim3(kBrickThreads), 0, st, b.sys,
-                       b.sys_cell_base, b.sys_brick_base, b.nsys, b.M, b.nat, b.cell_start, b.rec, b.wbig,
-                       b.pair_count, b.slots, b.slotsR, b.slot_cap, b.status, b.fb_queue);
+    if (b.count_block == 256)
+        hipLaunchKernelGGL((count_kernel<R, W, 256>), dim3((unsigned)b.nbricks), dim3(256), 0, st, b.sys,
+                           b.sys_cell_base, b.sys_brick_base, b.nsys, b.M, b.nat, b.cell_start, b.rec, b.wbig,
+                           b.pair_count, b.slots, b.slotsR, b.slot_cap, b.status, b.fb_queue);
+    else
+        hipLaunchKernelGGL((count_kernel<R, W, 512>), dim3((unsigned)b.nbricks), dim3(512), 0, st, b.sys,
+                           b.sys_cell_base, b.sys_brick_base, b.nsys, b.M, b.nat, b.cell_start, b.rec, b.wbig,
+                           b.pair_count, b.slots, b.slotsR, b.slot_cap, b.status, b.fb_queue);
     const unsigned nfb = (unsigned)std::min<int64_t>(b.nbricks, 512);
     hipLaunchKernelGGL((count_fallback_kernel<R, W>), dim3(nfb), dim3(kFallbackThreads), 0, st, b.sys,
                        b.sys_cell_base, b.sys_brick_base, b.nsys, b.M, b.nat, b.cell_start, b.rec, b.wbig,

@@ -93,14 +93,16 @@ def test_graph_replay_outgrows_capacity():
     _same(g, g2)
 
 
-@pytest.mark.parametrize("brick_atoms,fill_block,fill_atoms", [(64, 64, 4), (512, 128, 32), (1024, 512, 64),
-                                                                (2048, 1024, 8), (1536, 256, 16)])
-def test_tuning_does_not_change_results(brick_atoms, fill_block, fill_atoms):
+@pytest.mark.parametrize("brick_atoms,fill_block,fill_atoms,count_block",
+                         [(64, 64, 4, 0), (512, 128, 32, 256), (1024, 512, 64, 512), (2048, 1024, 8, 256),
+                          (1536, 256, 16, 512), (768, 256, 8, 256)])
+def test_tuning_does_not_change_results(brick_atoms, fill_block, fill_atoms, count_block):
     s = workloads.random_cubic(6000, density=0.05, seed=81)
     h = _lib.Handle()
-    h.set_tuning(brick_atoms, fill_block, fill_atoms)
+    h.set_tuning(brick_atoms, fill_block, fill_atoms, count_block)
     t = h.get_tuning()
-    assert (t.brick_atoms, t.fill_block, t.fill_atoms) == (brick_atoms, fill_block, fill_atoms)
+    assert (t.brick_atoms, t.fill_block, t.fill_atoms, t.count_block) == (brick_atoms, fill_block, fill_atoms,
+                                                                          count_block)
     nl = nlhip.neighbour_list(s.X, 5.0, s.cell, (True, False, True), want_R=True, handle=h)
     ref = oracle.neighbour_list(s.X, 5.0, s.cell, (True, False, True), want_R=True)
     _same(nl, ref)
@@ -114,6 +116,8 @@ def test_tuning_rejects_bad_values():
         h.set_tuning(1024, 100)
     with pytest.raises(_lib.NLHInvalidError):
         h.set_tuning(1024, 256, 12)
+    with pytest.raises(_lib.NLHInvalidError):
+        h.set_tuning(1024, 256, 8, 128)
 
 
 def test_autotune_keeps_results():
@@ -122,10 +126,12 @@ def test_autotune_keeps_results():
     bb = BatchBuilder()
     before = bb.run(systems, want_R=True)
     best = bb.autotune(X0, reps=1)
-    assert best.brick_atoms in (512, 1024, 1536) and best.fill_block in (128, 256, 512)
+    assert best.brick_atoms in (384, 768, 1536) and best.fill_block in (128, 256, 512)
+    assert best.count_block in (256, 512)
     assert best.fill_atoms in (4, 8, 16)
     t = bb.handle.get_tuning()
-    assert (t.brick_atoms, t.fill_block, t.fill_atoms) == (best.brick_atoms, best.fill_block, best.fill_atoms)
+    assert (t.brick_atoms, t.fill_block, t.fill_atoms, t.count_block) == (best.brick_atoms, best.fill_block,
+                                                                          best.fill_atoms, best.count_block)
     after = bb.rebuild(X0, want_R=True)
     _same(before, after)
     _check_oracle(after, systems, X0, offs, [0, 1, 64, 127])
