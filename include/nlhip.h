@@ -165,8 +165,8 @@ int nlh_batch_rebuild(nlh_handle *h, const double *X_batch, uint32_t flags, int6
  * workgroup, a multiple of 64 up to 1024.  fill_atoms: atoms per fill wave
  * (4, 8, 16, 32 or 64; the wave writes their pairs as one contiguous range;
  * default 8).  count_block: threads per count-pass brick workgroup, 256 or
- * 512, or 0 for automatic (512 for one system, 256 for a batch); a
- * 256-thread brick holds at most 768 expected atoms.
+ * 512, or 0 for the default (256); a 256-thread brick holds at most 768
+ * expected atoms (default brick_atoms 768).
  * Every configuration gives identical results; no reference counterpart
  * (KernelAbstractions picks workgroup sizes itself). */
 typedef struct {

@@ -109,7 +109,7 @@ struct nlh_handle {
     int device = 0;
     hipStream_t stream = nullptr;
     DevBuf X, tables, key, slot, cell_count, cell_start, tmp, rec, wbig, perm, cell_id,
-        cell_offsets, pair_count, fbq, inv_perm, slots, slotsR, pair_off, partials, sys_pair_base, first, i, j, S, R, status, Xsorted, nb_j, nb_R,
+        cell_offsets, pair_count, inv_perm, slots, slotsR, pair_off, partials, sys_pair_base, first, i, j, S, R, status, Xsorted, nb_j, nb_R,
         nb_S, nb_count;
     // system tables, one device buffer: SysDev[nsys], atom bases int64[nsys + 1],
     // cell bases int32[nsys + 1], brick bases int32[nsys + 1]
@@ -126,7 +126,6 @@ struct nlh_handle {
     int32_t M = 0;
     int ti_bytes = 4;
     bool built = false, materialized = false, have_R = false;
-    bool queue_clean = false;   // the count pass's brick queue is empty (status just cleared)
     int64_t npairs = 0;
     int64_t pair_capacity = 0;
     int32_t slot_cap = 0;       // neighbour slots per atom
@@ -151,11 +150,11 @@ struct nlh_handle {
     nlh::PipelineBufs graph_bufs;
     int64_t graph_launches = 0;
     // launch configuration (nlh_set_tuning / nlh_autotune)
-    nlh_tuning tune = {nlh::kBrickAtomBudget, 256, 8, 0};
+    nlh_tuning tune = {768, 256, 8, 0};
 
-    // count-pass workgroup: 8 waves per brick for one system, 4 for a batch
-    // (whose systems are mostly one small brick each), unless tuned
-    int count_block() const { return tune.count_block ? tune.count_block : (nsys > 1 ? 256 : 512); }
+    // count-pass workgroup: 4 waves per brick unless tuned (four 36 KB
+    // bricks share a CU; 8-wave bricks of 72 KB fit two)
+    int count_block() const { return tune.count_block ? tune.count_block : 256; }
 
     nlh::PipelineBufs bufs() const
     {
@@ -181,7 +180,6 @@ struct nlh_handle {
         b.cell_id_out = cell_id.p;
         b.cell_offsets_out = cell_offsets.p;
         b.pair_count = pair_count.as<int32_t>();
-        b.fb_queue = fbq.as<int32_t>();
         b.inv_perm = inv_perm.as<int32_t>();
         b.slots = slots.p;
         b.slotsR = slotsR.as<double>();
@@ -422,7 +420,6 @@ void assign_bricks(nlh_handle *h)
     }
     h->brick_base_host[nsys] = (int32_t)bricks;
     h->nbricks = bricks;
-    h->fbq.ensure(h->stream, (size_t)std::max<int64_t>(bricks, 1) * 4);
 }
 
 // Upload the system tables (sys, atom/cell/brick bases) as one copy through
@@ -525,15 +522,6 @@ void upload_positions(nlh_handle *h, const double *X, uint32_t flags)
 void reset_status(nlh_handle *h)
 {
     nlh::launch_clear_status(h->status.as<nlh::Status>(), h->stream);
-    h->queue_clean = true;
-}
-
-// Every count launch starts from an empty fallback-brick queue.
-void count_pass(nlh_handle *h, const nlh::PipelineBufs &b, bool want_R)
-{
-    if (!h->queue_clean) nlh::launch_reset_queue(b.status, h->stream);
-    nlh::launch_count(b, want_R, h->wide, h->stream);
-    h->queue_clean = false;
 }
 
 void mark(nlh_handle *h, int k)
@@ -652,7 +640,6 @@ void check_pair_buffers(const nlh_handle *h, bool want_R)
     ok = ok && h->pair_count.cap >= n1 * 4 && h->pair_off.cap >= (n1 + 1) * 8 && h->inv_perm.cap >= n1 * 4;
     ok = ok && h->first.cap >= (n1 + h->nsys) * ti && h->sys_pair_base.cap >= (size_t)(h->nsys + 1) * 8;
     ok = ok && h->i.cap >= cap * ti && h->j.cap >= cap * ti && h->S.cap >= cap * 3 * ti;
-    ok = ok && h->fbq.cap >= (size_t)std::max<int64_t>(h->nbricks, 1) * 4;
     if (want_R) ok = ok && h->slotsR.p && h->slotsR.cap >= ent * 24 && h->R.p && h->R.cap >= cap * 24;
     if (!ok) throw Error(NLH_EDEVICE, "internal: pair-stage buffers are smaller than the launch needs");
 }
@@ -685,7 +672,7 @@ void materialize(nlh_handle *h, uint32_t flags)
     check_pair_buffers(h, want_R);
     nlh::PipelineBufs b = h->bufs();
     mark(h, 1);
-    count_pass(h, b, want_R);
+    nlh::launch_count(b, want_R, h->wide, h->stream);
     mark(h, 2);
     nlh::launch_scan_pairs(b, h->stream);
     mark(h, 3);
@@ -705,7 +692,7 @@ void materialize(nlh_handle *h, uint32_t flags)
         ensure_slots(h, round_slot_cap(std::max(st.max_count, 1)), want_R);
         check_pair_buffers(h, want_R);
         b = h->bufs();
-        count_pass(h, b, want_R);
+        nlh::launch_count(b, want_R, h->wide, h->stream);
         nlh::launch_fill(b, want_R, h->wide, h->stream);
         HIPC(hipGetLastError());
         finish_status(h);
@@ -750,7 +737,7 @@ void capture_graph(nlh_handle *h, bool want_R)
         // kernel launches only between begin and end (nothing that takes the lock)
         std::unique_lock<std::shared_mutex> lk(g_capture_mutex);
         HIPC(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-        nlh::launch_clear_status(b.status, h->stream);   // also empties the brick queue
+        nlh::launch_clear_status(b.status, h->stream);
         nlh::launch_build(b, h->stream);
         nlh::launch_count(b, want_R, h->wide, h->stream);
         nlh::launch_scan_pairs(b, h->stream);
@@ -781,7 +768,6 @@ bool rebuild_graph(nlh_handle *h, bool want_R)
     if (!h->graph_exec || h->graph_want_R != want_R || std::memcmp(&b, &h->graph_bufs, sizeof b) != 0)
         capture_graph(h, want_R);
     HIPC(hipGraphLaunch(h->graph_exec, h->stream));
-    h->queue_clean = false;
     ++h->graph_launches;
     h->built = true;
     h->materialized = false;
@@ -942,7 +928,7 @@ int nlh_destroy(nlh_handle *h)
         });
         DevBuf *bufs[] = {&h->X,        &h->tables,    &h->key,    &h->slot,
                           &h->cell_count, &h->cell_start, &h->tmp,      &h->rec,          &h->wbig,   &h->perm,
-                          &h->cell_id,  &h->cell_offsets, &h->pair_count, &h->fbq, &h->inv_perm, &h->slots, &h->slotsR, &h->pair_off,   &h->partials,
+                          &h->cell_id,  &h->cell_offsets, &h->pair_count, &h->inv_perm, &h->slots, &h->slotsR, &h->pair_off,   &h->partials,
                           &h->sys_pair_base, &h->first, &h->i,          &h->j,            &h->S,      &h->R,
                           &h->status,   &h->Xsorted,   &h->nb_j,       &h->nb_R,         &h->nb_S,   &h->nb_count};
         for (DevBuf *b : bufs) b->release();

@@ -19,7 +19,6 @@ constexpr int kWavesPerPairsBlock = kPairsBlock / kWave;
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;             // elements per thread in a scan tile
 constexpr int kScanTile = kScanThreads * kScanItems;
-constexpr int kBrickAtomBudget = 1536;    // expected halo atoms per brick (LDS holds 2048)
 
 // One system of a batch (a single build is a batch of one).
 struct SysDev {
@@ -81,8 +80,7 @@ __host__ __device__ inline void unpack_wind(uint32_t w, int32_t &wx, int32_t &wy
 // Device-side status block, copied to pinned host memory once per build.
 struct Status {
     int64_t npairs;          // total pairs over the batch
-    int32_t fb_count;        // bricks queued for the global-memory count path
-    int32_t spare;
+    int64_t reserved;
     int32_t overflow;        // pair capacity exceeded: outputs not written
     int32_t bad_position;    // non-finite / out-of-range position seen
     int32_t wide;            // a cell shift does not fit a packed slot entry
@@ -184,7 +182,6 @@ struct PipelineBufs {
     void *cell_offsets_out;     // M + nsys, TI, 1-based per system
     // pairs
     int32_t *pair_count;
-    int32_t *fb_queue;       // bricks for the global-memory count path (nbricks)
     int32_t *inv_perm;          // nat: original -> sorted index
     void *slots;                // nat * slot_cap slot entries
     double *slotsR;             // 3 * nat * slot_cap (R requested)
@@ -206,7 +203,6 @@ struct PipelineBufs {
 void launch_clear_status(Status *status, hipStream_t st);
 void launch_build(const PipelineBufs &b, hipStream_t st);
 void launch_count(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st);
-void launch_reset_queue(Status *status, hipStream_t st);
 void launch_scan_pairs(const PipelineBufs &b, hipStream_t st);
 void launch_fill(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st);
 void launch_sorted_positions(const PipelineBufs &b, double *Xs, hipStream_t st);

@@ -334,8 +334,6 @@ struct BrickShape {
     static constexpr int halo = BT;       // halo cells, one per thread
     static constexpr int unroll = cap / BT;
 };
-constexpr int kFallbackThreads = 256;
-constexpr int kFallbackWaves = kFallbackThreads / kWave;
 constexpr uint32_t kPackZero = (512u) | (512u << 10) | (512u << 20);   // pack_wind(0, 0, 0)
 constexpr int kBrickChunks = 3;   // candidate chunks a lane holds in the brick pass (192 candidates)
 
@@ -654,14 +652,15 @@ __device__ __forceinline__ void test_home_atom(int nch, const double *cx, const 
 // Brick pass of the count: home atoms stream through the wave as scalars
 // (read out of one-per-lane registers), candidates sit one per lane in
 // registers, chunk by chunk; each (home atom, chunk) is one FP64 distance
-// test and one ballot.  Bricks that cannot take this path are queued for
-// count_fallback_kernel.
+// test and one ballot.  Bricks that cannot take this path (reach beyond one
+// cell, halo over the LDS, windings beyond the packed range) are counted
+// from global memory, one home cell per wave.
 template <bool WANT_R, bool WIDE, int BT>
 __global__ __launch_bounds__(BT) void count_kernel(
     const SysDev *__restrict__ sys, const int32_t *__restrict__ sys_cell_base, const int32_t *__restrict__ sys_brick_base,
     int nsys, int32_t M, int64_t nat, const int32_t *__restrict__ cell_start, const Rec *__restrict__ rec,
     const int32_t *__restrict__ wbig, int32_t *__restrict__ pair_count, void *__restrict__ slots,
-    double *__restrict__ slotsR, int32_t cap, Status *__restrict__ status, int32_t *__restrict__ fb_queue)
+    double *__restrict__ slotsR, int32_t cap, Status *__restrict__ status)
 {
     __shared__ BrickLDS<BT> L;
     __shared__ double Csh[9];
@@ -672,7 +671,10 @@ __global__ __launch_bounds__(BT) void count_kernel(
     if (threadIdx.x < 9) Csh[threadIdx.x] = B.sd->C[threadIdx.x];   // visible after brick_load's barrier
     const bool lds = B.fast && brick_load(A, B, L);
     if (!lds) {
-        if (threadIdx.x == 0) fb_queue[atomicAdd(&status->fb_count, 1)] = b;
+        const SlotArgs O{slots, slotsR, cap, status};
+        const int wv0 = sgpr((int)(threadIdx.x >> 6));
+        for (int hc = wv0; hc < B.nhome; hc += BrickShape<BT>::waves)
+            count_cell<WANT_R, WIDE>(A, *B.sd, brick_cell_id(B, hc), pair_count, O);
         return;
     }
     const SysDev &sd = *B.sd;
@@ -768,33 +770,6 @@ __global__ __launch_bounds__(BT) void count_kernel(
             if (lane < nh) pair_count[sd.atom_base + hj_l] = cnt_l | (narrow ? 0 : kRow8Bit);
         }
     }
-}
-
-// Global-memory count of the bricks the brick pass queued (reach beyond one
-// cell, halo over the LDS budget, windings beyond the packed range): one
-// home cell per wave.
-template <bool WANT_R, bool WIDE>
-__global__ __launch_bounds__(kFallbackThreads) void count_fallback_kernel(
-    const SysDev *__restrict__ sys, const int32_t *__restrict__ sys_cell_base, const int32_t *__restrict__ sys_brick_base,
-    int nsys, int32_t M, int64_t nat, const int32_t *__restrict__ cell_start, const Rec *__restrict__ rec,
-    const int32_t *__restrict__ wbig, int32_t *__restrict__ pair_count, void *__restrict__ slots,
-    double *__restrict__ slotsR, int32_t cap, Status *__restrict__ status, const int32_t *__restrict__ fb_queue)
-{
-    const int nq = status->fb_count;
-    const PairsArgs A{sys, sys_cell_base, nsys, M, nat, cell_start, rec, wbig};
-    const SlotArgs O{slots, slotsR, cap, status};
-    const int wv = sgpr((int)(threadIdx.x >> 6));
-    for (int q = blockIdx.x; q < nq; q += gridDim.x) {
-        Brick B;
-        brick_locate(A, sys_brick_base, fb_queue[q], 0, B);
-        for (int hc = wv; hc < B.nhome; hc += kFallbackWaves)
-            count_cell<WANT_R, WIDE>(A, *B.sd, brick_cell_id(B, hc), pair_count, O);
-    }
-}
-
-__global__ void reset_queue_kernel(Status *__restrict__ status)
-{
-    if (threadIdx.x == 0) status->fb_count = 0;
 }
 
 // Fill pass: atoms in their original order, one wave per 64 consecutive
@@ -978,22 +953,11 @@ static void count_launch(const PipelineBufs &b, hipStream_t st)
     if (b.count_block == 256)
         hipLaunchKernelGGL((count_kernel<R, W, 256>), dim3((unsigned)b.nbricks), dim3(256), 0, st, b.sys,
                            b.sys_cell_base, b.sys_brick_base, b.nsys, b.M, b.nat, b.cell_start, b.rec, b.wbig,
-                           b.pair_count, b.slots, b.slotsR, b.slot_cap, b.status, b.fb_queue);
+                           b.pair_count, b.slots, b.slotsR, b.slot_cap, b.status);
     else
         hipLaunchKernelGGL((count_kernel<R, W, 512>), dim3((unsigned)b.nbricks), dim3(512), 0, st, b.sys,
                            b.sys_cell_base, b.sys_brick_base, b.nsys, b.M, b.nat, b.cell_start, b.rec, b.wbig,
-                           b.pair_count, b.slots, b.slotsR, b.slot_cap, b.status, b.fb_queue);
-    const unsigned nfb = (unsigned)std::min<int64_t>(b.nbricks, 512);
-    hipLaunchKernelGGL((count_fallback_kernel<R, W>), dim3(nfb), dim3(kFallbackThreads), 0, st, b.sys,
-                       b.sys_cell_base, b.sys_brick_base, b.nsys, b.M, b.nat, b.cell_start, b.rec, b.wbig,
-                       b.pair_count, b.slots, b.slotsR, b.slot_cap, b.status, (const int32_t *)b.fb_queue);
-}
-
-// The fallback queue is cleared with the rest of the status at the start of
-// a build; a repeated count (after the buffers grew) clears it here first.
-void launch_reset_queue(Status *status, hipStream_t st)
-{
-    hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, st, status);
+                           b.pair_count, b.slots, b.slotsR, b.slot_cap, b.status);
 }
 
 void launch_count(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st)

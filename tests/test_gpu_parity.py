@@ -205,8 +205,8 @@ def test_golden_fixture(path):
 
 def test_materialize_twice():
     """Repeated materialisation of one cell list (with and without R) gives
-    the same pairs; the count pass's fallback-brick queue starts empty each
-    time (cells smaller than the cutoff send every brick to that queue)."""
+    the same pairs (cells smaller than the cutoff send every brick down the
+    count pass's global-memory path)."""
     X = 5.0 * workloads.uniform(19, 300).reshape(100, 3)
     for rc in (2.0, 1.2):
         cl = nlhip.build_cell_list(X, rc, np.eye(3) * 5.0, (True, True, True))
