@@ -162,9 +162,9 @@ int nlh_batch_rebuild(nlh_handle *h, const double *X_batch, uint32_t flags, int6
 /* Launch configuration of the build.  brick_atoms: expected atoms (halo
  * included) of one count-pass brick, which sets the brick shape of each
  * system (64..2048; the LDS holds 2048).  fill_block: threads per fill
- * workgroup, a multiple of 64 up to 1024.  fill_atoms: atoms per fill wave
- * (4, 8, 16, 32 or 64; the wave writes their pairs as one contiguous range;
- * default 8).  count_block: threads per count-pass brick workgroup, 256 or
+ * workgroup, a multiple of 64 up to 1024.  fill_pairs: output pairs per
+ * fill wave (128, 256 or 512; each wave writes one aligned tile of the
+ * pair arrays; default 256).  count_block: threads per count-pass brick workgroup, 256 or
  * 512, or 0 for the default (256); a 256-thread brick holds at most 768
  * expected atoms (default brick_atoms 768).
  * Every configuration gives identical results; no reference counterpart
@@ -172,7 +172,7 @@ int nlh_batch_rebuild(nlh_handle *h, const double *X_batch, uint32_t flags, int6
 typedef struct {
     int32_t brick_atoms;
     int32_t fill_block;
-    int32_t fill_atoms;
+    int32_t fill_pairs;
     int32_t count_block;
 } nlh_tuning;
 int nlh_set_tuning(nlh_handle *h, const nlh_tuning *t);
@@ -180,8 +180,7 @@ int nlh_get_tuning(nlh_handle *h, nlh_tuning *t);
 /* Time the current batch (set up by a previous build on this handle) under
  * each candidate configuration (count_block in {256, 512} x brick_atoms
  * in {384, 768, 1536} (768 at most for 256) x fill_block in {128, 256, 512}
- * x fill_atoms in {4, 8, 16}, reps timed
- * rebuilds each, HIP events on the
+ * x fill_pairs in {256, 512}, reps timed rebuilds each, HIP events on the
  * handle's stream), keep the fastest and return it in *best.  Leaves the
  * handle holding a build of X_batch under the winning configuration. */
 int nlh_autotune(nlh_handle *h, const double *X_batch, uint32_t flags, int32_t reps, nlh_tuning *best);

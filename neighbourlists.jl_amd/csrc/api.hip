@@ -109,7 +109,7 @@ struct nlh_handle {
     int device = 0;
     hipStream_t stream = nullptr;
     DevBuf X, tables, key, slot, cell_count, cell_start, tmp, rec, wbig, perm, cell_id,
-        cell_offsets, pair_count, inv_perm, slots, slotsR, pair_off, partials, sys_pair_base, first, i, j, S, R, status, Xsorted, nb_j, nb_R,
+        cell_offsets, pair_count, inv_perm, slots, slotsR, pair_off, tiles, partials, sys_pair_base, first, i, j, S, R, status, Xsorted, nb_j, nb_R,
         nb_S, nb_count;
     // system tables, one device buffer: SysDev[nsys], atom bases int64[nsys + 1],
     // cell bases int32[nsys + 1], brick bases int32[nsys + 1]
@@ -150,7 +150,7 @@ struct nlh_handle {
     nlh::PipelineBufs graph_bufs;
     int64_t graph_launches = 0;
     // launch configuration (nlh_set_tuning / nlh_autotune)
-    nlh_tuning tune = {768, 256, 8, 0};
+    nlh_tuning tune = {768, 256, 256, 0};
 
     // count-pass workgroup: 4 waves per brick unless tuned (four 36 KB
     // bricks share a CU; 8-wave bricks of 72 KB fit two)
@@ -185,6 +185,8 @@ struct nlh_handle {
         b.slotsR = slotsR.as<double>();
         b.slot_cap = slot_cap;
         b.pair_off = pair_off.as<int64_t>();
+        b.tile_atom = tiles.as<int32_t>();
+        b.tile_cap = (int64_t)(tiles.cap / 4);
         b.scan_partials = partials.as<int64_t>();
         b.sys_pair_base = sys_pair_base.as<int64_t>();
         b.first_out = first.p;
@@ -196,7 +198,7 @@ struct nlh_handle {
         b.status = status.as<nlh::Status>();
         b.ti_bytes = ti_bytes;
         b.fill_block = tune.fill_block;
-        b.fill_atoms = tune.fill_atoms;
+        b.fill_pairs = tune.fill_pairs;
         b.count_block = count_block();
         return b;
     }
@@ -600,6 +602,8 @@ void ensure_pair_outputs(nlh_handle *h, int64_t cap, bool want_R)
     c = std::min<int64_t>(c, (int64_t)(h->S.cap / (3 * ti)));
     if (want_R) c = std::min<int64_t>(c, (int64_t)(h->R.cap / 24));
     h->pair_capacity = c;
+    // fill tiles over the capacity at the smallest tile size (128 pairs)
+    h->tiles.ensure(h->stream, (size_t)(c / nlh::kMinFillPairs + 2) * 4);
 }
 
 void ensure_pair_buffers(nlh_handle *h)
@@ -640,6 +644,7 @@ void check_pair_buffers(const nlh_handle *h, bool want_R)
     ok = ok && h->pair_count.cap >= n1 * 4 && h->pair_off.cap >= (n1 + 1) * 8 && h->inv_perm.cap >= n1 * 4;
     ok = ok && h->first.cap >= (n1 + h->nsys) * ti && h->sys_pair_base.cap >= (size_t)(h->nsys + 1) * 8;
     ok = ok && h->i.cap >= cap * ti && h->j.cap >= cap * ti && h->S.cap >= cap * 3 * ti;
+    ok = ok && h->tiles.cap >= (cap / nlh::kMinFillPairs + 2) * 4;
     if (want_R) ok = ok && h->slotsR.p && h->slotsR.cap >= ent * 24 && h->R.p && h->R.cap >= cap * 24;
     if (!ok) throw Error(NLH_EDEVICE, "internal: pair-stage buffers are smaller than the launch needs");
 }
@@ -693,6 +698,7 @@ void materialize(nlh_handle *h, uint32_t flags)
         check_pair_buffers(h, want_R);
         b = h->bufs();
         nlh::launch_count(b, want_R, h->wide, h->stream);
+        nlh::launch_scan_pairs(b, h->stream);   // fill tiles over the grown capacity
         nlh::launch_fill(b, want_R, h->wide, h->stream);
         HIPC(hipGetLastError());
         finish_status(h);
@@ -794,8 +800,8 @@ void validate_tuning(const nlh_tuning &t)
         throw Error(NLH_EINVAL, "brick_atoms must be in [64, 2048] (the LDS holds 2048 halo atoms)");
     if (t.fill_block < 64 || t.fill_block > 1024 || t.fill_block % 64 != 0)
         throw Error(NLH_EINVAL, "fill_block must be a multiple of 64 in [64, 1024]");
-    if (t.fill_atoms != 4 && t.fill_atoms != 8 && t.fill_atoms != 16 && t.fill_atoms != 32 && t.fill_atoms != 64)
-        throw Error(NLH_EINVAL, "fill_atoms must be 4, 8, 16, 32 or 64");
+    if (t.fill_pairs != 128 && t.fill_pairs != 256 && t.fill_pairs != 512)
+        throw Error(NLH_EINVAL, "fill_pairs must be 128, 256 or 512");
     if (t.count_block != 0 && t.count_block != 256 && t.count_block != 512)
         throw Error(NLH_EINVAL, "count_block must be 0 (automatic), 256 or 512");
 }
@@ -928,7 +934,7 @@ int nlh_destroy(nlh_handle *h)
         });
         DevBuf *bufs[] = {&h->X,        &h->tables,    &h->key,    &h->slot,
                           &h->cell_count, &h->cell_start, &h->tmp,      &h->rec,          &h->wbig,   &h->perm,
-                          &h->cell_id,  &h->cell_offsets, &h->pair_count, &h->inv_perm, &h->slots, &h->slotsR, &h->pair_off,   &h->partials,
+                          &h->cell_id,  &h->cell_offsets, &h->pair_count, &h->inv_perm, &h->slots, &h->slotsR, &h->pair_off,   &h->tiles, &h->partials,
                           &h->sys_pair_base, &h->first, &h->i,          &h->j,            &h->S,      &h->R,
                           &h->status,   &h->Xsorted,   &h->nb_j,       &h->nb_R,         &h->nb_S,   &h->nb_count};
         for (DevBuf *b : bufs) b->release();
@@ -1240,12 +1246,12 @@ int nlh_autotune(nlh_handle *h, const double *X_batch, uint32_t flags, int32_t r
         try {
             static const int32_t bricks[] = {384, 768, 1536};
             static const int32_t fills[] = {128, 256, 512};
-            static const int32_t atoms[] = {4, 8, 16};
+            static const int32_t tiles[] = {256, 512};
             static const int32_t blocks[] = {256, 512};
             for (int32_t cb : blocks)
             for (int32_t ba : bricks)
                 for (int32_t fb : fills)
-                    for (int32_t fa : atoms) {
+                    for (int32_t fa : tiles) {
                     if (cb == 256 && ba > 768) continue;   // above what a 256-thread brick holds
                     apply_tuning(h, nlh_tuning{ba, fb, fa, cb});
                     rebuild_eager(h, flags & ~NLH_NO_SYNC);   // warm-up (and buffer growth)

@@ -133,7 +133,8 @@ __global__ __launch_bounds__(kScanThreads) void scan_cells_down_kernel(
 template <typename TI>
 __global__ __launch_bounds__(kScanThreads) void scan_pairs_down_kernel(
     const int32_t *__restrict__ counts, int64_t nat, const int64_t *__restrict__ partials, int64_t *__restrict__ pair_off,
-    int nsys, void *__restrict__ first_out, Status *__restrict__ status, int64_t capacity)
+    int nsys, void *__restrict__ first_out, Status *__restrict__ status, int64_t capacity,
+    int32_t *__restrict__ tile_atom, int64_t tile_cap, int tile_shift)
 {
     int32_t v[kScanItems];
     const int64_t base = (int64_t)blockIdx.x * kScanTile;
@@ -163,6 +164,11 @@ __global__ __launch_bounds__(kScanThreads) void scan_pairs_down_kernel(
         const int64_t a = b + q;
         if (a < nat) {
             pair_off[a] = run;
+            // fill tiles whose first pair is one of this atom's: the atom
+            // owning pair t << tile_shift
+            const int64_t tsz = (int64_t)1 << tile_shift;
+            for (int64_t t = (run + tsz - 1) >> tile_shift; t < tile_cap && (t << tile_shift) < run + v[q]; ++t)
+                tile_atom[t] = (int32_t)a;
             if (nsys == 1) store_ti<TI>(first_out, a, run + 1);
             if (a == nat - 1) {
                 pair_off[nat] = run + v[q];
@@ -327,7 +333,8 @@ void launch_scan_pairs(const PipelineBufs &b, hipStream_t st)
     NLH_TI_DISPATCH(b.ti_bytes,
                     hipLaunchKernelGGL(scan_pairs_down_kernel<TI>, dim3((unsigned)nb), dim3(kScanThreads), 0, st,
                                        b.pair_count, n, nb > 1 ? b.scan_partials : nullptr, b.pair_off, b.nsys,
-                                       b.first_out, b.status, b.pair_capacity));
+                                       b.first_out, b.status, b.pair_capacity, b.tile_atom, b.tile_cap,
+                                       fill_tile_shift(b.fill_pairs)));
     if (b.nsys > 1)
         NLH_TI_DISPATCH(b.ti_bytes,
                         hipLaunchKernelGGL(batch_first_kernel<TI>, dim3(blocks_for(n + b.nsys + 1, 256)), dim3(256),

@@ -19,6 +19,10 @@ constexpr int kWavesPerPairsBlock = kPairsBlock / kWave;
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;             // elements per thread in a scan tile
 constexpr int kScanTile = kScanThreads * kScanItems;
+constexpr int kMinFillPairs = 128;        // smallest fill tile (pairs per fill wave)
+
+// log2 of the fill tile (pairs per fill wave: 128, 256 or 512)
+inline int fill_tile_shift(int fill_pairs) { return fill_pairs == 128 ? 7 : (fill_pairs == 512 ? 9 : 8); }
 
 // One system of a batch (a single build is a batch of one).
 struct SysDev {
@@ -187,6 +191,8 @@ struct PipelineBufs {
     double *slotsR;             // 3 * nat * slot_cap (R requested)
     int32_t slot_cap;
     int64_t *pair_off;          // nat + 1
+    int32_t *tile_atom;         // tile_cap: atom owning pair t * fill_pairs
+    int64_t tile_cap;
     int64_t *scan_partials;     // scan block sums
     int64_t *sys_pair_base;     // nsys + 1 (device)
     void *first_out;            // nat + nsys, TI, 1-based per system
@@ -196,7 +202,7 @@ struct PipelineBufs {
     Status *status;
     int ti_bytes;
     int fill_block;          // threads per fill workgroup (multiple of 64)
-    int fill_atoms;          // atoms per fill wave (4 to 64)
+    int fill_pairs;          // output pairs per fill wave (128, 256 or 512)
     int count_block;         // threads per count-pass brick workgroup (256 or 512)
 };
 

@@ -772,17 +772,22 @@ __global__ __launch_bounds__(BT) void count_kernel(
     }
 }
 
-// Fill pass: atoms in their original order, one wave per 64 consecutive
-// atoms and one lane per pair, so the (i, j, S[, R]) stores of a wave cover
-// one contiguous range of the output arrays (fill_pairs_kernel!,
-// gpu_kernels.jl:159-180).  Each lane takes kFillUnroll pairs per round (64
-// apart), issuing all their slot loads before any store.
-constexpr int kFillUnroll = 4;
+// Fill pass (fill_pairs_kernel!, gpu_kernels.jl:159-180): one wave per tile
+// of TP = 64 TW consecutive output pairs, one lane per pair.  Tiles start at
+// multiples of TP, so every store instruction covers one aligned 64-pair
+// block of i, j, S (and R): the stores write whole cache lines, and lines
+// shared between waves occur only at tile edges.  A wave issues the slot
+// loads of all its TW blocks before its first store.
+//
+// tile_atom[t] (written by the pair scan) is the atom owning pair t TP.  The
+// wave loads the offsets and counts of that atom and the 63 after it, one
+// atom per lane, and finds the atom of each of its pairs by a binary search
+// over the lanes.  A tile whose pairs reach past those 64 atoms (possible
+// only when atoms without pairs lie in between) is filled by a per-lane
+// search of pair_off instead.
 
-// Slot entries of the fill, read raw in one round and decoded in the next:
-// the loads carry no branch (8 bytes are read for every format) and their
-// results are not touched until the round after, so a wave keeps two rounds
-// of entry loads in flight.
+// Slot entries, read raw (8 bytes for every narrow format, no branch) and
+// decoded only after all of a wave's loads are issued.
 template <bool WIDE>
 struct RawEntry;
 template <>
@@ -804,7 +809,7 @@ __device__ __forceinline__ RawEntry<WIDE> load_entry(const void *__restrict__ sl
     } else {
         // 8-byte rows: entry rs + r; 4-byte rows: 4-byte entry 2 rs + r of
         // the same row (the 8 bytes read stay inside the row: 4 r + 8 <= 8 cap)
-        const int64_t byte = row8 ? 8 * (rs + r) : 4 * (2 * rs + r);
+        const int64_t byte = 8 * rs + ((int64_t)r << (row8 ? 3 : 2));
         __builtin_memcpy(&x.e, reinterpret_cast<const char *>(slots) + byte, 8);
         x.w8 = row8;
     }
@@ -829,116 +834,129 @@ __device__ __forceinline__ void decode_entry(const RawEntry<WIDE> &x, int32_t &j
     }
 }
 
+// Decode and store one pair at output position pos.
 template <bool WANT_R, bool WIDE, typename TI>
-struct FillRound {
-    RawEntry<WIDE> e[kFillUnroll];
-    int64_t is[kFillUnroll], ridx[kFillUnroll];
-};
-
-// Locate and load the kFillUnroll pairs h0 + 64 u + lane of a wave's range.
-template <bool WANT_R, bool WIDE, typename TI>
-__device__ __forceinline__ void fill_load(FillRound<WANT_R, WIDE, TI> &F, int h0, int tot, int ex, int r8, int64_t row,
-                                          int64_t il, const void *__restrict__ slots)
+__device__ __forceinline__ void fill_store_one(const RawEntry<WIDE> &e, int32_t is, int64_t ridx, int64_t pos,
+                                               bool live, const double *__restrict__ slotsR, TI *__restrict__ i_out,
+                                               TI *__restrict__ j_out, TI *__restrict__ S_out,
+                                               double *__restrict__ R_out)
 {
-    const int lane = lane_id();
-#pragma unroll
-    for (int u = 0; u < kFillUnroll; ++u) {
-        const int L = h0 + u * kWave + lane;
-        int s = 0;   // atom of this pair: largest s with ex[s] <= L
-#pragma unroll
-        for (int step = 32; step >= 1; step >>= 1) {
-            const int e2 = __shfl(ex, s + step, kWave);
-            if (e2 <= L) s += step;
-        }
-        const int r = L - __shfl(ex, s, kWave);
-        const bool w8 = __shfl(r8, s, kWave) != 0;
-        const int64_t rs = (int64_t)shfl64((uint64_t)row, s);
-        F.is[u] = (int64_t)shfl64((uint64_t)il, s);
-        F.ridx[u] = rs + r;   // entry index in 8-byte (or 16-byte) units; R rows use it too
-        // lanes past the wave's pairs read entry 0 of the last atom's row (in
-        // bounds, never stored): no branch around the load
-        F.e[u] = load_entry<WIDE>(slots, rs, L < tot ? r : 0, w8);
-    }
-}
-
-template <bool WANT_R, bool WIDE, typename TI>
-__device__ __forceinline__ void fill_store(const FillRound<WANT_R, WIDE, TI> &F, int h0, int tot, int64_t off0,
-                                           const double *__restrict__ slotsR, TI *__restrict__ i_out,
-                                           TI *__restrict__ j_out, TI *__restrict__ S_out, double *__restrict__ R_out)
-{
-    const int lane = lane_id();
-#pragma unroll
-    for (int u = 0; u < kFillUnroll; ++u) {
-        const int L = h0 + u * kWave + lane;
-        int32_t j, Sx, Sy, Sz;
-        decode_entry<WIDE>(F.e[u], j, Sx, Sy, Sz);
-        if (L < tot) {
-            const int64_t pos = off0 + L;
-            i_out[pos] = (TI)F.is[u];
-            j_out[pos] = (TI)(j + 1);
-            TI *S = S_out + 3 * pos;
-            S[0] = (TI)Sx;
-            S[1] = (TI)Sy;
-            S[2] = (TI)Sz;
-            if (WANT_R) {
-                const double *Rs = slotsR + 3 * F.ridx[u];
-                double *Rd = R_out + 3 * pos;
-                Rd[0] = Rs[0];
-                Rd[1] = Rs[1];
-                Rd[2] = Rs[2];
-            }
+    int32_t j, Sx, Sy, Sz;
+    decode_entry<WIDE>(e, j, Sx, Sy, Sz);   // outside the branch: it covers the stores only
+    if (live) {
+        i_out[pos] = (TI)is;
+        j_out[pos] = (TI)(j + 1);
+        TI *S = S_out + 3 * pos;
+        S[0] = (TI)Sx;
+        S[1] = (TI)Sy;
+        S[2] = (TI)Sz;
+        if (WANT_R) {
+            const double *Rs = slotsR + 3 * ridx;
+            double *Rd = R_out + 3 * pos;
+            Rd[0] = Rs[0];
+            Rd[1] = Rs[1];
+            Rd[2] = Rs[2];
         }
     }
 }
 
-// AW atoms per wave (4 to 64): fewer atoms per wave make waves
-// shorter-lived, so the output ranges being written at any moment lie closer
-// together.
-template <bool WANT_R, bool WIDE, typename TI, int AW>
+template <bool WANT_R, bool WIDE, typename TI, int TW>
 __global__ __launch_bounds__(1024) void fill_kernel(
     const int64_t *__restrict__ sys_atom_base, int nsys, int64_t nat, const int32_t *__restrict__ pair_count,
-    const int64_t *__restrict__ pair_off, const void *__restrict__ slots, const double *__restrict__ slotsR,
-    const Status *__restrict__ status, int32_t cap, int64_t pair_capacity, TI *__restrict__ i_out,
-    TI *__restrict__ j_out, TI *__restrict__ S_out, double *__restrict__ R_out)
+    const int64_t *__restrict__ pair_off, const int32_t *__restrict__ tile_atom, int64_t tile_cap,
+    const void *__restrict__ slots, const double *__restrict__ slotsR, const Status *__restrict__ status, int32_t cap,
+    int64_t pair_capacity, TI *__restrict__ i_out, TI *__restrict__ j_out, TI *__restrict__ S_out,
+    double *__restrict__ R_out)
 {
-    if (status->npairs > pair_capacity || status->max_count > cap || (!WIDE && status->wide)) return;
+    constexpr int TP = TW * kWave;
+    constexpr int32_t kBeyond = 1 << 30;   // window position past every pair of the tile
     const int lane = lane_id();
-    const int64_t a0 = (((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * AW;
-    if (a0 >= nat) return;
-    const int64_t a = a0 + lane;
-    const bool va = lane < AW && a < nat;
-    const int32_t pc = va ? pair_count[a] : 0;
-    const int cnt = pc & kCountMask;
-    const int r8 = (pc & kRow8Bit) ? 1 : 0;
-    const int64_t off = va ? pair_off[a] : 0;
-    // slot row of atom a (rows are in original order): entry index (8-byte
-    // units) of its first entry; a 4-byte row is addressed in 4-byte units
-    // from the same start
-    const int64_t row = va ? a * (int64_t)cap : 0;
-    int64_t il = a + 1;   // 1-based system-local index of atom a
-    if (nsys > 1 && va) il = a - sys_atom_base[find_sys(sys_atom_base, nsys, a)] + 1;
-    const int incl = wave_incl_scan(cnt);
-    const int tot = sgpr(__shfl(incl, kWave - 1, kWave));
-    const int ex = incl - cnt;
-    const int64_t off0 = readlane64(off, 0);   // the wave's pairs are [off0, off0 + tot)
-    constexpr int kRound = kFillUnroll * kWave;
-    // software pipeline over two buffers: the next round's slot loads are in
-    // flight while this round's pairs are stored (no register copies between
-    // rounds, which would wait for the loads).  The empty asm keeps each
-    // round's loads ahead of the previous round's stores: one memory counter
-    // covers both, so a load issued after stores would make its wait drain
-    // them too.
-    FillRound<WANT_R, WIDE, TI> A, B;
-    if (tot > 0) fill_load(A, 0, tot, ex, r8, row, il, slots);
-    for (int h0 = 0; h0 < tot; h0 += 2 * kRound) {
-        const int h1 = h0 + kRound;
-        if (h1 < tot) fill_load(B, h1, tot, ex, r8, row, il, slots);
-        __asm__ volatile("" ::: "memory");
-        fill_store(A, h0, tot, off0, slotsR, i_out, j_out, S_out, R_out);
-        if (h1 >= tot) break;
-        if (h1 + kRound < tot) fill_load(A, h1 + kRound, tot, ex, r8, row, il, slots);
-        __asm__ volatile("" ::: "memory");
-        fill_store(B, h1, tot, off0, slotsR, i_out, j_out, S_out, R_out);
+    // tiles in block order: the waves in flight write one advancing front of
+    // the output (an XCD-contiguous block order measured much slower)
+    const int64_t w = (int64_t)blockIdx.x * (blockDim.x / kWave) + sgpr((int)(threadIdx.x / kWave));   // wave-uniform
+    const int64_t Q0 = w * TP;
+    const Status st = *status;
+    // read with the status (index clamped to the table: waves past the
+    // pairs return below without using it)
+    const int32_t a0s = tile_atom[w < tile_cap ? w : tile_cap - 1];
+    __asm__ volatile("" ::"s"(a0s));   // keeps the load ahead of the branch: its latency overlaps the status's
+    const int64_t A0 = a0s;
+    if ((st.npairs > pair_capacity) | (st.max_count > cap) | (!WIDE & (st.wide != 0)) | (Q0 >= st.npairs)) return;
+    const int nlive = (int)(st.npairs - Q0 < TP ? st.npairs - Q0 : TP);   // pairs of this tile
+    // the window: atom A0 + k on lane k, its run of pairs [fo, fo + cnt)
+    // relative to Q0, clamped to +-kBeyond
+    const int64_t ak = A0 + lane;
+    const bool vk = ak < nat;
+    const int64_t akc = vk ? ak : nat - 1;
+    const int32_t pck = pair_count[akc];
+    const int64_t rel = pair_off[akc] - Q0;
+    const int32_t cntk = pck & kCountMask;
+    const int32_t fo = vk ? (int32_t)(rel < -kBeyond ? -kBeyond : (rel > kBeyond ? kBeyond : rel)) : kBeyond;
+    const int64_t endk = rel + cntk;
+    const int32_t endc = vk ? (int32_t)(endk > kBeyond ? kBeyond : endk) : kBeyond;
+    const int32_t wend = __builtin_amdgcn_readlane(endc, kWave - 1);   // end of the window's runs
+    if (wend >= nlive) {
+        const uint64_t row8 = __ballot(vk && (pck & kRow8Bit) != 0);
+        const int32_t fo0 = __builtin_amdgcn_readlane(fo, 0);   // <= 0: atom A0 owns pair Q0
+        const bool batch = nsys > 1;
+        int32_t il = 0;   // batches: 1-based system-local index of the lane's atom
+        if (batch && vk) il = (int32_t)(ak - sys_atom_base[find_sys(sys_atom_base, nsys, ak)] + 1);
+        RawEntry<WIDE> e[TW];
+        int32_t is[TW];
+        int64_t ridx[TW];   // used only with R
+#pragma unroll
+        for (int u = 0; u < TW; ++u) {
+            const int q = u * kWave + lane;
+            // largest window lane s whose run starts at or before q
+            int s = 0, f = fo0;
+#pragma unroll
+            for (int step = 32; step >= 1; step >>= 1) {
+                const int f2 = __shfl(fo, s + step, kWave);
+                if (f2 <= q) {
+                    s += step;
+                    f = f2;
+                }
+            }
+            const bool live = q < nlive;
+            // pairs past the tile's end read entry 0 of row 0 (in bounds, never
+            // stored): no branch around the load
+            const int r = live ? q - f : 0;
+            const int64_t rs = live ? (A0 + s) * (int64_t)cap : 0;
+            const bool w8 = ((row8 >> s) & 1) != 0;
+            is[u] = batch ? __shfl(il, s, kWave) : (int32_t)(A0 + s + 1);
+            ridx[u] = rs + r;
+            e[u] = load_entry<WIDE>(slots, rs, r, w8);
+        }
+#pragma unroll
+        for (int u = 0; u < TW; ++u) {
+            const int q = u * kWave + lane;
+            fill_store_one<WANT_R, WIDE, TI>(e[u], is[u], ridx[u], Q0 + q, q < nlive, slotsR, i_out,
+                                             j_out, S_out, R_out);
+        }
+        return;
+    }
+    // the tile's pairs reach past the window: each lane searches pair_off for
+    // the atom of its pair, between A0 and the atom owning the next tile's
+    // first pair
+    const int64_t hi0 = Q0 + TP < st.npairs ? (int64_t)tile_atom[w + 1] : nat - 1;
+    for (int u = 0; u < TW; ++u) {
+        const int q = u * kWave + lane;
+        if (q < nlive) {
+            const int64_t P = Q0 + q;
+            int64_t lo = A0, hi = hi0;   // largest atom in [lo, hi] whose run starts at or before P
+            while (lo < hi) {
+                const int64_t mid = (lo + hi + 1) >> 1;
+                if (pair_off[mid] <= P) lo = mid;
+                else hi = mid - 1;
+            }
+            const int32_t pc = pair_count[lo];
+            const int r = (int)(P - pair_off[lo]);
+            const int64_t rs = lo * (int64_t)cap;
+            int32_t isv = (int32_t)(lo + 1);
+            if (nsys > 1) isv = (int32_t)(lo - sys_atom_base[find_sys(sys_atom_base, nsys, lo)] + 1);
+            const RawEntry<WIDE> e = load_entry<WIDE>(slots, rs, r, (pc & kRow8Bit) != 0);
+            fill_store_one<WANT_R, WIDE, TI>(e, isv, rs + r, P, true, slotsR, i_out, j_out, S_out, R_out);
+        }
     }
 }
 
@@ -972,36 +990,36 @@ void launch_count(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st)
     }
 }
 
-template <bool R, bool W, typename TI, int AW>
+template <bool R, bool W, typename TI, int TW>
 static void fill_launch(const PipelineBufs &b, hipStream_t st)
 {
+    // one wave per tile of the pair capacity (waves past the pairs return)
     const int bs = b.fill_block;
-    const int64_t waves = (b.nat + AW - 1) / AW;
+    const int64_t waves = (b.pair_capacity + TW * kWave - 1) / (TW * kWave);
+    if (waves <= 0) return;
     const unsigned nblk = (unsigned)((waves * kWave + bs - 1) / bs);
-    hipLaunchKernelGGL((fill_kernel<R, W, TI, AW>), dim3(nblk), dim3(bs), 0, st, b.sys_atom_base, b.nsys, b.nat,
-                       b.pair_count, b.pair_off, b.slots, b.slotsR, b.status, b.slot_cap, b.pair_capacity,
-                       (TI *)b.i_out, (TI *)b.j_out, (TI *)b.S_out, b.R_out);
+    hipLaunchKernelGGL((fill_kernel<R, W, TI, TW>), dim3(nblk), dim3(bs), 0, st, b.sys_atom_base, b.nsys, b.nat,
+                       b.pair_count, b.pair_off, b.tile_atom, b.tile_cap, b.slots, b.slotsR, b.status, b.slot_cap,
+                       b.pair_capacity, (TI *)b.i_out, (TI *)b.j_out, (TI *)b.S_out, b.R_out);
 }
 
 template <bool R, bool W, typename TI>
-static void fill_atoms(const PipelineBufs &b, hipStream_t st)
+static void fill_tiles(const PipelineBufs &b, hipStream_t st)
 {
-    if (b.fill_atoms == 4) fill_launch<R, W, TI, 4>(b, st);
-    else if (b.fill_atoms == 8) fill_launch<R, W, TI, 8>(b, st);
-    else if (b.fill_atoms == 16) fill_launch<R, W, TI, 16>(b, st);
-    else if (b.fill_atoms == 32) fill_launch<R, W, TI, 32>(b, st);
-    else fill_launch<R, W, TI, 64>(b, st);
+    if (b.fill_pairs == 128) fill_launch<R, W, TI, 2>(b, st);
+    else if (b.fill_pairs == 512) fill_launch<R, W, TI, 8>(b, st);
+    else fill_launch<R, W, TI, 4>(b, st);
 }
 
 template <typename TI>
 static void fill_dispatch(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st)
 {
     if (want_R) {
-        if (wide) fill_atoms<true, true, TI>(b, st);
-        else fill_atoms<true, false, TI>(b, st);
+        if (wide) fill_tiles<true, true, TI>(b, st);
+        else fill_tiles<true, false, TI>(b, st);
     } else {
-        if (wide) fill_atoms<false, true, TI>(b, st);
-        else fill_atoms<false, false, TI>(b, st);
+        if (wide) fill_tiles<false, true, TI>(b, st);
+        else fill_tiles<false, false, TI>(b, st);
     }
 }
 

@@ -68,12 +68,12 @@ class Geometry(C.Structure):
 
 class Tuning(C.Structure):
     """nlh_tuning: launch configuration (identical results for every value)."""
-    _fields_ = [("brick_atoms", C.c_int32), ("fill_block", C.c_int32), ("fill_atoms", C.c_int32),
+    _fields_ = [("brick_atoms", C.c_int32), ("fill_block", C.c_int32), ("fill_pairs", C.c_int32),
                 ("count_block", C.c_int32)]
 
     def __repr__(self):
         return (f"Tuning(brick_atoms={self.brick_atoms}, fill_block={self.fill_block}, "
-                f"fill_atoms={self.fill_atoms}, count_block={self.count_block})")
+                f"fill_pairs={self.fill_pairs}, count_block={self.count_block})")
 
 
 class PairListView(C.Structure):
@@ -271,8 +271,8 @@ class Handle:
     def synchronize(self):
         check(lib().nlh_synchronize(self._h), "synchronize")
 
-    def set_tuning(self, brick_atoms: int, fill_block: int, fill_atoms: int = 8, count_block: int = 0):
-        t = Tuning(int(brick_atoms), int(fill_block), int(fill_atoms), int(count_block))
+    def set_tuning(self, brick_atoms: int, fill_block: int, fill_pairs: int = 256, count_block: int = 0):
+        t = Tuning(int(brick_atoms), int(fill_block), int(fill_pairs), int(count_block))
         check(lib().nlh_set_tuning(self._h, C.byref(t)), "set_tuning")
 
     def get_tuning(self) -> "Tuning":

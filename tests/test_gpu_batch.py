@@ -93,15 +93,15 @@ def test_graph_replay_outgrows_capacity():
     _same(g, g2)
 
 
-@pytest.mark.parametrize("brick_atoms,fill_block,fill_atoms,count_block",
-                         [(64, 64, 4, 0), (512, 128, 32, 256), (1024, 512, 64, 512), (2048, 1024, 8, 256),
-                          (1536, 256, 16, 512), (768, 256, 8, 256)])
-def test_tuning_does_not_change_results(brick_atoms, fill_block, fill_atoms, count_block):
+@pytest.mark.parametrize("brick_atoms,fill_block,fill_pairs,count_block",
+                         [(64, 64, 128, 0), (512, 128, 512, 256), (1024, 512, 256, 512), (2048, 1024, 128, 256),
+                          (1536, 256, 512, 512), (768, 256, 256, 256)])
+def test_tuning_does_not_change_results(brick_atoms, fill_block, fill_pairs, count_block):
     s = workloads.random_cubic(6000, density=0.05, seed=81)
     h = _lib.Handle()
-    h.set_tuning(brick_atoms, fill_block, fill_atoms, count_block)
+    h.set_tuning(brick_atoms, fill_block, fill_pairs, count_block)
     t = h.get_tuning()
-    assert (t.brick_atoms, t.fill_block, t.fill_atoms, t.count_block) == (brick_atoms, fill_block, fill_atoms,
+    assert (t.brick_atoms, t.fill_block, t.fill_pairs, t.count_block) == (brick_atoms, fill_block, fill_pairs,
                                                                           count_block)
     nl = nlhip.neighbour_list(s.X, 5.0, s.cell, (True, False, True), want_R=True, handle=h)
     ref = oracle.neighbour_list(s.X, 5.0, s.cell, (True, False, True), want_R=True)
@@ -115,9 +115,9 @@ def test_tuning_rejects_bad_values():
     with pytest.raises(_lib.NLHInvalidError):
         h.set_tuning(1024, 100)
     with pytest.raises(_lib.NLHInvalidError):
-        h.set_tuning(1024, 256, 12)
+        h.set_tuning(1024, 256, 200)
     with pytest.raises(_lib.NLHInvalidError):
-        h.set_tuning(1024, 256, 8, 128)
+        h.set_tuning(1024, 256, 256, 128)
 
 
 def test_autotune_keeps_results():
@@ -128,10 +128,10 @@ def test_autotune_keeps_results():
     best = bb.autotune(X0, reps=1)
     assert best.brick_atoms in (384, 768, 1536) and best.fill_block in (128, 256, 512)
     assert best.count_block in (256, 512)
-    assert best.fill_atoms in (4, 8, 16)
+    assert best.fill_pairs in (256, 512)
     t = bb.handle.get_tuning()
-    assert (t.brick_atoms, t.fill_block, t.fill_atoms, t.count_block) == (best.brick_atoms, best.fill_block,
-                                                                          best.fill_atoms, best.count_block)
+    assert (t.brick_atoms, t.fill_block, t.fill_pairs, t.count_block) == (best.brick_atoms, best.fill_block,
+                                                                          best.fill_pairs, best.count_block)
     after = bb.rebuild(X0, want_R=True)
     _same(before, after)
     _check_oracle(after, systems, X0, offs, [0, 1, 64, 127])

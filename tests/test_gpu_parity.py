@@ -217,3 +217,39 @@ def test_materialize_twice():
         for nl in (a, b, c):
             assert np.array_equal(nl.i, ref.i) and np.array_equal(nl.j, ref.j) and np.array_equal(nl.S, ref.S)
         assert np.array_equal(np.asarray(b.R).view(np.int64), ref.R.view(np.int64))
+
+
+def _sparse_with_dimers(ngrid, ndimer, seed):
+    """Atoms on a grid of spacing 6 (no neighbour within 5) plus a partner at
+    distance 1 for `ndimer` of them, shuffled: most atoms own no pairs, so
+    a tile of output pairs spans thousands of atoms."""
+    rng = np.random.default_rng(seed)
+    g = int(round(ngrid ** (1 / 3)))
+    ax = 6.0 * np.arange(g) + 3.0
+    grid = np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3)
+    pick = rng.choice(grid.shape[0], ndimer, replace=False)
+    X = np.concatenate([grid, grid[pick] + np.array([1.0, 0.0, 0.0])])
+    return X[rng.permutation(X.shape[0])], np.eye(3) * (6.0 * g)
+
+
+@pytest.mark.parametrize("fill_pairs", [128, 256, 512])
+def test_fill_tiles_over_many_empty_atoms(fill_pairs):
+    """Fill tiles whose pairs lie far apart in atom order (the per-lane
+    search path) and tiles next to it (the 64-atom window path)."""
+    h = nlhip._lib.Handle()
+    h.set_tuning(768, 256, fill_pairs)
+    X, C = _sparse_with_dimers(8000, 700, seed=91)
+    print(f"sparse dimers fill_pairs={fill_pairs}", flush=True)
+    for pbc in [(True, True, True), (False, True, False)]:
+        nl = nlhip.neighbour_list(X, 5.0, C, pbc, want_R=True, handle=h)
+        ref = oracle.neighbour_list(X, 5.0, C, pbc, want_R=True)
+        assert ref.npairs == 1400
+        _same(nl, ref)
+    # dense clusters among empty atoms: windows that just cover, or just miss, a tile
+    rng = np.random.default_rng(92)
+    Xg, Cg = _sparse_with_dimers(4096, 0, seed=93)
+    cl = 30.0 + 1.5 * rng.standard_normal((300, 3))
+    Xc = np.concatenate([Xg, cl])[rng.permutation(Xg.shape[0] + 300)]
+    nl = nlhip.neighbour_list(Xc, 5.0, Cg, (True, True, True), want_R=True, handle=h)
+    ref = oracle.neighbour_list(Xc, 5.0, Cg, (True, True, True), want_R=True)
+    _same(nl, ref)
