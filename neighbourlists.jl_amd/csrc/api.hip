@@ -186,7 +186,7 @@ struct nlh_handle {
         b.slot_cap = slot_cap;
         b.pair_off = pair_off.as<int64_t>();
         b.tile_atom = tiles.as<int32_t>();
-        b.tile_cap = (int64_t)(tiles.cap / 4);
+        b.tile_cap = (int64_t)(tiles.cap / 8);
         b.scan_partials = partials.as<int64_t>();
         b.sys_pair_base = sys_pair_base.as<int64_t>();
         b.first_out = first.p;
@@ -603,7 +603,7 @@ void ensure_pair_outputs(nlh_handle *h, int64_t cap, bool want_R)
     if (want_R) c = std::min<int64_t>(c, (int64_t)(h->R.cap / 24));
     h->pair_capacity = c;
     // fill tiles over the capacity at the smallest tile size (128 pairs)
-    h->tiles.ensure(h->stream, (size_t)(c / nlh::kMinFillPairs + 2) * 4);
+    h->tiles.ensure(h->stream, (size_t)(c / nlh::kMinFillPairs + 2) * 8);
 }
 
 void ensure_pair_buffers(nlh_handle *h)
@@ -644,7 +644,7 @@ void check_pair_buffers(const nlh_handle *h, bool want_R)
     ok = ok && h->pair_count.cap >= n1 * 4 && h->pair_off.cap >= (n1 + 1) * 8 && h->inv_perm.cap >= n1 * 4;
     ok = ok && h->first.cap >= (n1 + h->nsys) * ti && h->sys_pair_base.cap >= (size_t)(h->nsys + 1) * 8;
     ok = ok && h->i.cap >= cap * ti && h->j.cap >= cap * ti && h->S.cap >= cap * 3 * ti;
-    ok = ok && h->tiles.cap >= (cap / nlh::kMinFillPairs + 2) * 4;
+    ok = ok && h->tiles.cap >= (cap / nlh::kMinFillPairs + 2) * 8;
     if (want_R) ok = ok && h->slotsR.p && h->slotsR.cap >= ent * 24 && h->R.p && h->R.cap >= cap * 24;
     if (!ok) throw Error(NLH_EDEVICE, "internal: pair-stage buffers are smaller than the launch needs");
 }

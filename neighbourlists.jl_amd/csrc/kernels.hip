@@ -164,11 +164,14 @@ __global__ __launch_bounds__(kScanThreads) void scan_pairs_down_kernel(
         const int64_t a = b + q;
         if (a < nat) {
             pair_off[a] = run;
-            // fill tiles whose first pair is one of this atom's: the atom
-            // owning pair t << tile_shift
+            // fill tiles whose first pair is one of this atom's: entry t is
+            // the atom owning pair t << tile_shift and the number of its
+            // pairs before that one
             const int64_t tsz = (int64_t)1 << tile_shift;
-            for (int64_t t = (run + tsz - 1) >> tile_shift; t < tile_cap && (t << tile_shift) < run + v[q]; ++t)
-                tile_atom[t] = (int32_t)a;
+            for (int64_t t = (run + tsz - 1) >> tile_shift; t < tile_cap && (t << tile_shift) < run + v[q]; ++t) {
+                tile_atom[2 * t] = (int32_t)a;
+                tile_atom[2 * t + 1] = (int32_t)((t << tile_shift) - run);
+            }
             if (nsys == 1) store_ti<TI>(first_out, a, run + 1);
             if (a == nat - 1) {
                 pair_off[nat] = run + v[q];

@@ -191,7 +191,7 @@ struct PipelineBufs {
     double *slotsR;             // 3 * nat * slot_cap (R requested)
     int32_t slot_cap;
     int64_t *pair_off;          // nat + 1
-    int32_t *tile_atom;         // tile_cap: atom owning pair t * fill_pairs
+    int32_t *tile_atom;         // 2 tile_cap: atom owning pair t * fill_pairs, its pairs before it
     int64_t tile_cap;
     int64_t *scan_partials;     // scan block sums
     int64_t *sys_pair_base;     // nsys + 1 (device)

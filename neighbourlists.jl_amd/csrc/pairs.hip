@@ -779,12 +779,13 @@ __global__ __launch_bounds__(BT) void count_kernel(
 // shared between waves occur only at tile edges.  A wave issues the slot
 // loads of all its TW blocks before its first store.
 //
-// tile_atom[t] (written by the pair scan) is the atom owning pair t TP.  The
-// wave loads the offsets and counts of that atom and the 63 after it, one
-// atom per lane, and finds the atom of each of its pairs by a binary search
-// over the lanes.  A tile whose pairs reach past those 64 atoms (possible
-// only when atoms without pairs lie in between) is filled by a per-lane
-// search of pair_off instead.
+// The pair scan writes, per tile t, the atom A0 owning pair t TP and how
+// many of its pairs come before that one; the next tile's entry bounds the
+// tile's atoms from above (A1).  When the tile spans at most 64 atoms the
+// wave loads their counts, one atom per lane, rebuilds their run starts by
+// a scan, and finds the atom of each of its pairs by a binary search over
+// the lanes.  A tile spanning more atoms (possible only when atoms without
+// pairs lie in between) is filled by a per-lane search of pair_off instead.
 
 // Slot entries, read raw (8 bytes for every narrow format, no branch) and
 // decoded only after all of a wave's loads are issued.
@@ -834,7 +835,10 @@ __device__ __forceinline__ void decode_entry(const RawEntry<WIDE> &x, int32_t &j
     }
 }
 
-// Decode and store one pair at output position pos.
+// Decode and store one pair at output position pos.  The stores are
+// non-temporal: the output streams past the caches, which keep holding the
+// slot rows the count pass wrote and the fill reads (with cached stores the
+// 1M-atom fill ran at 147 us, with these at 103 us).
 template <bool WANT_R, bool WIDE, typename TI>
 __device__ __forceinline__ void fill_store_one(const RawEntry<WIDE> &e, int32_t is, int64_t ridx, int64_t pos,
                                                bool live, const double *__restrict__ slotsR, TI *__restrict__ i_out,
@@ -844,18 +848,18 @@ __device__ __forceinline__ void fill_store_one(const RawEntry<WIDE> &e, int32_t 
     int32_t j, Sx, Sy, Sz;
     decode_entry<WIDE>(e, j, Sx, Sy, Sz);   // outside the branch: it covers the stores only
     if (live) {
-        i_out[pos] = (TI)is;
-        j_out[pos] = (TI)(j + 1);
+        __builtin_nontemporal_store((TI)is, i_out + pos);
+        __builtin_nontemporal_store((TI)(j + 1), j_out + pos);
         TI *S = S_out + 3 * pos;
-        S[0] = (TI)Sx;
-        S[1] = (TI)Sy;
-        S[2] = (TI)Sz;
+        __builtin_nontemporal_store((TI)Sx, S);
+        __builtin_nontemporal_store((TI)Sy, S + 1);
+        __builtin_nontemporal_store((TI)Sz, S + 2);
         if (WANT_R) {
             const double *Rs = slotsR + 3 * ridx;
             double *Rd = R_out + 3 * pos;
-            Rd[0] = Rs[0];
-            Rd[1] = Rs[1];
-            Rd[2] = Rs[2];
+            __builtin_nontemporal_store(Rs[0], Rd);
+            __builtin_nontemporal_store(Rs[1], Rd + 1);
+            __builtin_nontemporal_store(Rs[2], Rd + 2);
         }
     }
 }
@@ -876,54 +880,73 @@ __global__ __launch_bounds__(1024) void fill_kernel(
     const int64_t w = (int64_t)blockIdx.x * (blockDim.x / kWave) + sgpr((int)(threadIdx.x / kWave));   // wave-uniform
     const int64_t Q0 = w * TP;
     const Status st = *status;
-    // read with the status (index clamped to the table: waves past the
-    // pairs return below without using it)
-    const int32_t a0s = tile_atom[w < tile_cap ? w : tile_cap - 1];
-    __asm__ volatile("" ::"s"(a0s));   // keeps the load ahead of the branch: its latency overlaps the status's
-    const int64_t A0 = a0s;
+    // the atoms owning this tile's first pair and the next tile's, read with
+    // the status (indexes clamped to the table, which holds at least two
+    // entries: waves past the pairs return below without using them)
+    const int64_t wc = w + 1 < tile_cap ? w : tile_cap - 2;
+    const int32_t a0s = tile_atom[2 * wc], skip0 = tile_atom[2 * wc + 1], a1s = tile_atom[2 * wc + 2];
+    __asm__ volatile("" ::"s"(a0s), "s"(skip0), "s"(a1s));   // keeps the loads ahead of the branch: their latency overlaps the status's
     if ((st.npairs > pair_capacity) | (st.max_count > cap) | (!WIDE & (st.wide != 0)) | (Q0 >= st.npairs)) return;
     const int nlive = (int)(st.npairs - Q0 < TP ? st.npairs - Q0 : TP);   // pairs of this tile
-    // the window: atom A0 + k on lane k, its run of pairs [fo, fo + cnt)
-    // relative to Q0, clamped to +-kBeyond
-    const int64_t ak = A0 + lane;
-    const bool vk = ak < nat;
-    const int64_t akc = vk ? ak : nat - 1;
-    const int32_t pck = pair_count[akc];
-    const int64_t rel = pair_off[akc] - Q0;
-    const int32_t cntk = pck & kCountMask;
-    const int32_t fo = vk ? (int32_t)(rel < -kBeyond ? -kBeyond : (rel > kBeyond ? kBeyond : rel)) : kBeyond;
-    const int64_t endk = rel + cntk;
-    const int32_t endc = vk ? (int32_t)(endk > kBeyond ? kBeyond : endk) : kBeyond;
-    const int32_t wend = __builtin_amdgcn_readlane(endc, kWave - 1);   // end of the window's runs
-    if (wend >= nlive) {
-        const uint64_t row8 = __ballot(vk && (pck & kRow8Bit) != 0);
-        const int32_t fo0 = __builtin_amdgcn_readlane(fo, 0);   // <= 0: atom A0 owns pair Q0
+    const int64_t A0 = a0s;
+    const int64_t A1 = Q0 + TP < st.npairs ? (int64_t)a1s : nat - 1;   // every pair of the tile is owned in [A0, A1]
+    if (A1 - A0 < kWave) {
+        // the window: atom A0 + k on lane k < span, its run of pairs
+        // [fo, fo + cnt) relative to Q0 (the exclusive scan of the counts,
+        // less the skip0 pairs of atom A0 before the tile); other lanes hold
+        // kBeyond
+        const int span = (int)(A1 - A0) + 1;
+        const int64_t ak = A0 + lane;
+        const bool vk = lane < span;
+        const int32_t pck = vk ? pair_count[ak] : 0;
+        const int32_t cntk = pck & kCountMask;
+        int32_t incl = cntk;
+        for (int o = 1; o < span; o <<= 1) {
+            const int32_t t = __shfl_up(incl, o, kWave);
+            if (lane >= o) incl += t;
+        }
+        // runs past the tile end keep their exact start (<= TP + 64 cap)
+        const int32_t fo = vk ? incl - cntk - skip0 : kBeyond;
+        const uint64_t row8 = __ballot((pck & kRow8Bit) != 0);
+        const int32_t fo0 = -skip0;   // atom A0 owns pair Q0
         const bool batch = nsys > 1;
         int32_t il = 0;   // batches: 1-based system-local index of the lane's atom
         if (batch && vk) il = (int32_t)(ak - sys_atom_base[find_sys(sys_atom_base, nsys, ak)] + 1);
+        // pair q = 64 u + lane belongs to the largest window lane s[u] whose
+        // run starts at or before q: one binary search per block, all TW
+        // searches advanced together (TW independent lane reads per step),
+        // over just enough steps for the tile's span of atoms
+        int top = 0;
+        while (2 * top < span) top = top ? 2 * top : 1;   // 2 top >= span: s reaches span - 1
+        int s[TW], f[TW];
+#pragma unroll
+        for (int u = 0; u < TW; ++u) {
+            s[u] = 0;
+            f[u] = fo0;
+        }
+        for (int step = top; step >= 1; step >>= 1) {
+#pragma unroll
+            for (int u = 0; u < TW; ++u) {
+                const int f2 = __shfl(fo, s[u] + step, kWave);
+                if (f2 <= u * kWave + lane) {
+                    s[u] += step;
+                    f[u] = f2;
+                }
+            }
+        }
         RawEntry<WIDE> e[TW];
         int32_t is[TW];
         int64_t ridx[TW];   // used only with R
 #pragma unroll
         for (int u = 0; u < TW; ++u) {
             const int q = u * kWave + lane;
-            // largest window lane s whose run starts at or before q
-            int s = 0, f = fo0;
-#pragma unroll
-            for (int step = 32; step >= 1; step >>= 1) {
-                const int f2 = __shfl(fo, s + step, kWave);
-                if (f2 <= q) {
-                    s += step;
-                    f = f2;
-                }
-            }
             const bool live = q < nlive;
             // pairs past the tile's end read entry 0 of row 0 (in bounds, never
             // stored): no branch around the load
-            const int r = live ? q - f : 0;
-            const int64_t rs = live ? (A0 + s) * (int64_t)cap : 0;
-            const bool w8 = ((row8 >> s) & 1) != 0;
-            is[u] = batch ? __shfl(il, s, kWave) : (int32_t)(A0 + s + 1);
+            const int r = live ? q - f[u] : 0;
+            const int64_t rs = live ? (A0 + s[u]) * (int64_t)cap : 0;
+            const bool w8 = ((row8 >> s[u]) & 1) != 0;
+            is[u] = batch ? __shfl(il, s[u], kWave) : (int32_t)(A0 + s[u] + 1);
             ridx[u] = rs + r;
             e[u] = load_entry<WIDE>(slots, rs, r, w8);
         }
@@ -935,10 +958,9 @@ __global__ __launch_bounds__(1024) void fill_kernel(
         }
         return;
     }
-    // the tile's pairs reach past the window: each lane searches pair_off for
-    // the atom of its pair, between A0 and the atom owning the next tile's
-    // first pair
-    const int64_t hi0 = Q0 + TP < st.npairs ? (int64_t)tile_atom[w + 1] : nat - 1;
+    // the tile's pairs span more than 64 atoms: each lane searches pair_off
+    // between A0 and A1 for the atom of its pair
+    const int64_t hi0 = A1;
     for (int u = 0; u < TW; ++u) {
         const int q = u * kWave + lane;
         if (q < nlive) {
