@@ -204,6 +204,7 @@ def _served(nreq: int, seed: int = 11) -> dict:
     for r in reqs:
         r.arrival = 0.0
     srv = serve.Server()
+    srv.serve(warm, realtime=False)     # warm, as the plain path's handle is
     served_sat = srv.serve(reqs, realtime=False)
     srv.close()
     so, po, ss = serve.summarize(served_open), serve.summarize(plain_open), serve.summarize(served_sat)

@@ -86,7 +86,8 @@ struct DevBuf {
         HIPC(hipStreamSynchronize(st));
         if (p) HIPD(hipFree(p));
         p = nullptr;
-        const size_t nb = std::max(bytes + bytes / 8, (size_t)256);
+        // a quarter of headroom: builds of varying size regrow rarely
+        const size_t nb = std::max(bytes + bytes / 4, (size_t)256);
         HIPD(hipMalloc(&p, nb));
         cap = nb;
         if (zero) {
@@ -120,6 +121,10 @@ struct nlh_handle {
     nlh::Status *status_host = nullptr;   // pinned
     void *staging = nullptr;              // pinned upload staging for system tables
     size_t staging_cap = 0;
+    void *xstage = nullptr;               // pinned upload staging for host positions
+    size_t xstage_cap = 0;
+    hipEvent_t xstage_ev = nullptr;       // recorded after the last upload out of xstage
+    bool xstage_pending = false;
     // state of the last build
     int nsys = 0;
     int64_t nat = 0;
@@ -509,6 +514,32 @@ void setup_batch(nlh_handle *h, int32_t nsys, const int64_t *atom_offsets, const
     upload_tables(h);
 }
 
+// Host positions go through the handle's pinned staging buffer: a copy
+// from pageable memory is staged by the runtime itself, and such copies from
+// several threads at once were measured to wait on each other for
+// milliseconds.  The buffer is reused once the previous upload out of it is
+// done (its event).
+void stage_positions(nlh_handle *h, const double *X, size_t bytes)
+{
+    if (h->xstage_pending) {
+        HIPC(hipEventSynchronize(h->xstage_ev));
+        h->xstage_pending = false;
+    }
+    if (bytes > h->xstage_cap) {
+        if (h->xstage) HIPD(hipHostFree(h->xstage));
+        h->xstage = nullptr;
+        h->xstage_cap = 0;
+        const size_t nb = std::max(bytes + bytes / 4, (size_t)4096);
+        HIPD(hipHostMalloc(&h->xstage, nb, hipHostMallocDefault));
+        h->xstage_cap = nb;
+    }
+    if (!h->xstage_ev) HIPD(hipEventCreateWithFlags(&h->xstage_ev, hipEventDisableTiming));
+    std::memcpy(h->xstage, X, bytes);
+    HIPC(hipMemcpyAsync(h->X.p, h->xstage, bytes, hipMemcpyHostToDevice, h->stream));
+    HIPC(hipEventRecord(h->xstage_ev, h->stream));
+    h->xstage_pending = true;
+}
+
 void upload_positions(nlh_handle *h, const double *X, uint32_t flags)
 {
     if (h->nat > 0 && X == nullptr) throw Error(NLH_EINVAL, "X is NULL");
@@ -516,7 +547,7 @@ void upload_positions(nlh_handle *h, const double *X, uint32_t flags)
         h->X_dev = X;
     } else {
         h->X.ensure(h->stream, (size_t)std::max<int64_t>(h->nat, 1) * 24);
-        if (h->nat > 0) HIPC(hipMemcpyAsync(h->X.p, X, (size_t)h->nat * 24, hipMemcpyHostToDevice, h->stream));
+        if (h->nat > 0) stage_positions(h, X, (size_t)h->nat * 24);
         h->X_dev = h->X.as<double>();
     }
 }
@@ -941,6 +972,8 @@ int nlh_destroy(nlh_handle *h)
         device_wide([&] {
             if (h->status_host) (void)hipHostFree(h->status_host);
             if (h->staging) (void)hipHostFree(h->staging);
+            if (h->xstage) (void)hipHostFree(h->xstage);
+            if (h->xstage_ev) (void)hipEventDestroy(h->xstage_ev);
             if (h->stream) (void)hipStreamDestroy(h->stream);
         });
         delete h;

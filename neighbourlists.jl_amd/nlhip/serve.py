@@ -18,8 +18,12 @@ keeps the device busy:
   device at once;
 * residency: a large system keeps its positions and its cell list on the
   device in a handle of its own; a rebuild request uploads the new
-  positions into the resident buffer and replays the captured pipeline
-  (nlh_batch_rebuild).
+  positions into the resident buffer (through the handle's pinned staging)
+  and replays the captured pipeline (nlh_batch_rebuild).  At most
+  `max_residents` systems stay resident: a new one then takes over the
+  handle of the least recently used idle resident (device buffers only
+  grow, so a steady stream allocates nothing; device frees synchronise the
+  whole device, so they happen only under HBM-budget pressure).
 
 Every result equals the single-system build of the same input bit for bit
 (the batch and replay paths emit in the reference traversal order), which
@@ -37,7 +41,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import NLH_X_DEVICE, DeviceArray, Handle, check, geometry, lib
+from ._lib import Handle, check, geometry, lib
 from .api import Int32, PairList
 from .batch import BatchBuilder, pack_systems
 from .workloads import System
@@ -93,31 +97,44 @@ def estimate_bytes(nat: int, cell, cutoff: float, ti_bytes: int = 4) -> int:
 
 
 class _Resident:
-    """A large system kept on the device: positions, handle (cell list,
-    pair list, captured pipeline)."""
+    """A large system kept on the device in a handle of its own: positions
+    (the handle's position buffer, written through its pinned staging),
+    cell list, pair list and captured pipeline.  `recycled` is the (handle,
+    bytes held) of an evicted resident to build on."""
 
-    def __init__(self, req: Request, int_type):
+    def __init__(self, req: Request, int_type, recycled=None):
         self.system_id = req.system_id
         self.nat = req.nat
         self.geom = geometry(req.cell, req.cutoff, req.pbc, _lib.ti_bytes_of(int_type))
         self.cell, self.pbc, self.cutoff = req.cell, req.pbc, req.cutoff
-        self.handle = Handle()
-        self.X = DeviceArray(np.ascontiguousarray(req.X, dtype=np.float64))
         self.bytes = estimate_bytes(req.nat, req.cell, req.cutoff, _lib.ti_bytes_of(int_type))
+        self.handle = None
+        if recycled is not None:
+            self.handle, held = recycled
+            self.bytes = max(self.bytes, held)      # its buffers never shrink
+        if self.handle is None:
+            self.handle = Handle()
         self.lock = threading.Lock()
         self.built = False
 
-    def build(self, X: np.ndarray | None) -> int:
+    def build(self, X: np.ndarray) -> int:
+        """First build, or a rebuild on new positions (upload into the
+        resident buffer, replay of the captured pipeline)."""
         n = C.c_int64()
-        if X is not None:
-            self.X.upload(X)
+        X = np.ascontiguousarray(X, dtype=np.float64)
         if self.built:
-            check(lib().nlh_batch_rebuild(self.handle.ptr, self.X.ptr, NLH_X_DEVICE, C.byref(n)), "rebuild")
+            check(lib().nlh_batch_rebuild(self.handle.ptr, X.ctypes.data, 0, C.byref(n)), "rebuild")
         else:
-            check(lib().nlh_neighbour_list(self.handle.ptr, self.X.ptr, self.nat, C.byref(self.geom), NLH_X_DEVICE,
+            check(lib().nlh_neighbour_list(self.handle.ptr, X.ctypes.data, self.nat, C.byref(self.geom), 0,
                                            C.byref(n)), "neighbour_list")
             self.built = True
         return int(n.value)
+
+    def detach(self):
+        """Hand the handle over to a new resident."""
+        out = (self.handle, self.bytes)
+        self.handle = None
+        return out
 
     def result(self, X_host: np.ndarray) -> PairList:
         from .api import _collect_pairs
@@ -126,8 +143,8 @@ class _Resident:
         return _collect_pairs(self.handle, self.geom, X_host, int(v.npairs), False, None)
 
     def free(self):
-        self.X.free()
-        self.handle.close()
+        if self.handle is not None:
+            self.handle.close()
 
 
 class Server:
@@ -141,17 +158,22 @@ class Server:
                   are resident (own handle, positions and cell list kept on
                   the device, rebuilt by graph replay)
     batch_atoms   atoms per batched build
+    max_residents resident systems kept; beyond it a new one recycles the
+                  least recently used idle resident's device buffers (a
+                  resident being rebuilt is not idle, so up to
+                  max_residents + workers - 1 can be held)
     keep_results  copy each result back to the host (PairList) on completion
     """
 
     def __init__(self, hbm_budget: int = 64 << 30, workers: int = 3, small_atoms: int = 65536,
-                 batch_atoms: int = 1 << 21, int_type=Int32, keep_results: bool = False):
+                 batch_atoms: int = 1 << 21, int_type=Int32, keep_results: bool = False, max_residents: int = 32):
         self.budget = int(hbm_budget)
         self.nworkers = int(workers)
         self.small_atoms = int(small_atoms)
         self.batch_atoms = int(batch_atoms)
         self.int_type = int_type
         self.keep = keep_results
+        self.max_residents = max(int(max_residents), 1)
         self.residents: OrderedDict[int, _Resident] = OrderedDict()
         self.resident_bytes = 0
         self.inflight_bytes = 0
@@ -163,27 +185,33 @@ class Server:
         self._free_workers = threading.Semaphore(0)
         self._threads: list[threading.Thread] = []
         self._errors: list[BaseException] = []
+        # one batch builder per worker slot, kept across serve() calls: its
+        # device buffers only grow, so a warm server allocates nothing
+        self._builders: list[BatchBuilder | None] = [None] * self.nworkers
         self.t0 = 0.0
 
     # ------------------------------------------------------------------ workers
 
-    def _worker(self):
-        local = {"bb": None}
+    def _worker(self, slot: int):
+        local = {"slot": slot}
         self._free_workers.release()
         while True:
             item = self._work.get()
             if item is None:
                 return
             try:
-                self._run_item(item, local)
+                done = self._run_item(item, local)
             except BaseException as e:   # reported by serve()
                 self._errors.append(e)
-                for r in item[1]:
-                    self._done.put(("error", r, 0))
-            finally:
-                self._free_workers.release()
+                done = [("error", r, 0) for r in item[1]]
+            # free the slot first: the scheduler, woken by the completions,
+            # then finds this worker available
+            self._free_workers.release()
+            for d in done:
+                self._done.put(d)
 
-    def _run_item(self, item, local):
+    def _run_item(self, item, local) -> list:
+        """Run one unit of work; returns its completions."""
         kind, reqs, nbytes = item
         start = time.perf_counter() - self.t0
         if kind == "large":
@@ -191,14 +219,13 @@ class Server:
             res = self._resident_for(r, nbytes)
             nbytes = 0                       # the reservation became the resident's
             with res.lock:
-                P = res.build(np.ascontiguousarray(r.X, dtype=np.float64) if (r.rebuild or res.built) else None)
+                P = res.build(r.X)
                 out = res.result(r.X) if self.keep else None
             end = time.perf_counter() - self.t0
-            self._done.put(("ok", Completion(r.rid, r.system_id, P, r.arrival, start, end, 1, out), nbytes))
-            return
-        if local["bb"] is None:
-            local["bb"] = BatchBuilder(self.int_type)
-        bb: BatchBuilder = local["bb"]
+            return [("ok", Completion(r.rid, r.system_id, P, r.arrival, start, end, 1, out), nbytes)]
+        bb = self._builders[local["slot"]]
+        if bb is None:
+            bb = self._builders[local["slot"]] = BatchBuilder(self.int_type)
         systems = [System(r.X, r.cell, r.pbc, r.cutoff) for r in reqs]
         bb.run(systems, copy=False)
         po = np.zeros(len(reqs) + 1, dtype=np.int64)
@@ -208,10 +235,12 @@ class Server:
             X, _ = pack_systems(systems)
             res = bb.collect(X, False)
         end = time.perf_counter() - self.t0
+        done = []
         for s, r in enumerate(reqs):
             out = res.pairlist(s) if res is not None else None
-            self._done.put(("ok", Completion(r.rid, r.system_id, int(po[s + 1] - po[s]), r.arrival, start, end,
-                                             len(reqs), out), nbytes if s == 0 else 0))
+            done.append(("ok", Completion(r.rid, r.system_id, int(po[s + 1] - po[s]), r.arrival, start, end,
+                                          len(reqs), out), nbytes if s == 0 else 0))
+        return done
 
     def _resident_for(self, r: Request, reserved: int) -> _Resident:
         """The resident entry of r's system; a new one takes over the bytes
@@ -224,7 +253,27 @@ class Server:
                 if res.nat != r.nat:
                     raise ValueError(f"rebuild of system {r.system_id} with a different atom count")
                 return res
-        res = _Resident(r, self.int_type)
+            # at the resident limit, the least recently used idle resident
+            # hands its device buffers to the new one
+            recycled = None
+            if len(self.residents) >= self.max_residents:
+                for sid in list(self.residents):
+                    old = self.residents[sid]
+                    if old.lock.acquire(blocking=False):
+                        try:
+                            del self.residents[sid]
+                            self.resident_bytes -= old.bytes
+                            self.evictions += 1
+                            recycled = old.detach()
+                        finally:
+                            old.lock.release()
+                        break
+        try:
+            res = _Resident(r, self.int_type, recycled)
+        except BaseException:
+            if recycled is not None:
+                recycled[0].close()
+            raise
         with self._lock:
             self.residents[r.system_id] = res
             self.inflight_bytes -= reserved
@@ -261,7 +310,10 @@ class Server:
             if estimate_bytes(r.nat, r.cell, r.cutoff) > self.budget:
                 raise _lib.NLHMemoryError(_lib.NLH_ENOMEM, f"request {r.rid} ({r.nat} atoms) alone exceeds the "
                                                            f"HBM budget of {self.budget} bytes")
-        self._threads = [threading.Thread(target=self._worker, daemon=True) for _ in range(self.nworkers)]
+        # a fresh count of free workers for this call's threads (the previous
+        # call's threads have exited)
+        self._free_workers = threading.Semaphore(0)
+        self._threads = [threading.Thread(target=self._worker, args=(i,), daemon=True) for i in range(self.nworkers)]
         for t in self._threads:
             t.start()
         out: list[Completion] = []
@@ -270,24 +322,27 @@ class Server:
         n = len(order)
         self.t0 = time.perf_counter()
         inflight_systems = set()
+
+        def complete(d):
+            st, c, nb = d
+            if st == "error":
+                raise self._errors[0]
+            with self._lock:
+                self.inflight_bytes -= nb       # completions release memory
+            inflight_systems.discard(c.system_id)
+            out.append(c)
+
         try:
             while len(out) < n:
                 now = time.perf_counter() - self.t0
                 while k < n and (not realtime or order[k].arrival <= now):
                     waiting.append(order[k])
                     k += 1
-                # completions release memory
                 while True:
                     try:
-                        st, c, nb = self._done.get_nowait()
+                        complete(self._done.get_nowait())
                     except queue.Empty:
                         break
-                    if st == "error":
-                        raise self._errors[0]
-                    with self._lock:
-                        self.inflight_bytes -= nb
-                    inflight_systems.discard(c.system_id)
-                    out.append(c)
                 # dispatch while a worker is free
                 dispatched = False
                 while waiting and self._free_workers.acquire(blocking=False):
@@ -297,11 +352,16 @@ class Server:
                         break
                     self._work.put(item)
                     dispatched = True
-                if not dispatched:
+                if not dispatched and len(out) < n:
+                    # sleep until the next completion or the next arrival
+                    # (bounded, so the loop also re-checks admission)
+                    t = 2e-3
                     if k < n and realtime:
-                        time.sleep(min(max(order[k].arrival - (time.perf_counter() - self.t0), 0.0), 2e-4))
-                    else:
-                        time.sleep(2e-5)
+                        t = min(t, max(order[k].arrival - (time.perf_counter() - self.t0), 0.0))
+                    try:
+                        complete(self._done.get(timeout=t) if t > 0 else self._done.get_nowait())
+                    except queue.Empty:
+                        pass
         finally:
             for _ in self._threads:
                 self._work.put(None)
@@ -357,6 +417,10 @@ class Server:
                 res.free()
             self.residents.clear()
             self.resident_bytes = 0
+            for bb in self._builders:
+                if bb is not None:
+                    bb.handle.close()
+            self._builders = [None] * self.nworkers
 
 
 def serve_plain(requests: list[Request], *, realtime: bool = True, int_type=Int32,

@@ -24,11 +24,17 @@ def _same(a, b):
     assert np.array_equal(a.S, b.S) and np.array_equal(a.first, b.first)
 
 
-def test_alone_vs_mixed():
+@pytest.mark.parametrize("max_residents", [32, 2])
+def test_alone_vs_mixed(max_residents):
+    # max_residents=2: new large systems take over evicted residents' buffers
     reqs = _stream(48, seed=21, nmax=30000)
     assert any(r.rebuild for r in reqs) and any(r.nat >= 2048 for r in reqs)
-    srv = serve.Server(workers=3, small_atoms=2048, batch_atoms=20000, keep_results=True)
+    srv = serve.Server(workers=3, small_atoms=2048, batch_atoms=20000, keep_results=True,
+                       max_residents=max_residents)
     mixed = srv.serve(reqs, realtime=False)
+    if max_residents == 2:
+        # residents in use by the other workers are not recycled
+        assert srv.evictions > 0 and len(srv.residents) <= 2 + 3 - 1
     srv.close()
     alone = serve.serve_plain(reqs, realtime=False, keep_results=True)
     assert [c.rid for c in mixed] == [r.rid for r in reqs]
