@@ -110,7 +110,7 @@ struct nlh_handle {
     int device = 0;
     hipStream_t stream = nullptr;
     DevBuf X, tables, key, slot, cell_count, cell_start, tmp, rec, wbig, perm, cell_id,
-        cell_offsets, pair_count, inv_perm, slots, slotsR, pair_off, tiles, partials, sys_pair_base, first, i, j, S, R, status, Xsorted, nb_j, nb_R,
+        cell_offsets, pair_count, inv_perm, slots, slotsR, pair_off, tiles, lookback, sys_pair_base, first, i, j, S, R, status, Xsorted, nb_j, nb_R,
         nb_S, nb_count;
     // system tables, one device buffer: SysDev[nsys], atom bases int64[nsys + 1],
     // cell bases int32[nsys + 1], brick bases int32[nsys + 1]
@@ -133,6 +133,7 @@ struct nlh_handle {
     bool built = false, materialized = false, have_R = false;
     int64_t npairs = 0;
     int64_t pair_capacity = 0;
+    bool pair_lookback_clean = false;   // the pair scan's descriptors are zero
     int32_t slot_cap = 0;       // neighbour slots per atom
     bool wide = false;          // 16-byte slot entries (cell shifts beyond +-511)
     const double *X_dev = nullptr;
@@ -192,7 +193,7 @@ struct nlh_handle {
         b.pair_off = pair_off.as<int64_t>();
         b.tile_atom = tiles.as<int32_t>();
         b.tile_cap = (int64_t)(tiles.cap / 8);
-        b.scan_partials = partials.as<int64_t>();
+        b.lookback = lookback.as<uint64_t>();
         b.sys_pair_base = sys_pair_base.as<int64_t>();
         b.first_out = first.p;
         b.i_out = i.p;
@@ -508,7 +509,7 @@ void setup_batch(nlh_handle *h, int32_t nsys, const int64_t *atom_offsets, const
     h->perm.ensure(h->stream, n1 * ti);
     h->cell_id.ensure(h->stream, n1 * ti);
     h->cell_offsets.ensure(h->stream, (M1 + nsys) * ti);
-    h->partials.ensure(h->stream, (std::max(M1, n1) / nlh::kScanTile + 2) * sizeof(int64_t));
+    h->lookback.ensure(h->stream, (size_t)nlh::lookback_entries((int64_t)M1, (int64_t)n1) * sizeof(uint64_t));
     h->status.ensure(h->stream, sizeof(nlh::Status));
 
     upload_tables(h);
@@ -552,9 +553,21 @@ void upload_positions(nlh_handle *h, const double *X, uint32_t flags)
     }
 }
 
+// Clears the status and every scan descriptor.
 void reset_status(nlh_handle *h)
 {
-    nlh::launch_clear_status(h->status.as<nlh::Status>(), h->stream);
+    nlh::launch_clear_status(h->bufs(), h->stream);
+    h->pair_lookback_clean = true;
+}
+
+// The pair scan, on descriptors that start from zero: cleared with the
+// status, or here when the scan runs again without a status clear (a second
+// materialisation, or the repeat after growing the outputs).
+void pair_scan(nlh_handle *h, const nlh::PipelineBufs &b)
+{
+    if (!h->pair_lookback_clean) nlh::launch_clear_pair_lookback(b, h->stream);
+    nlh::launch_scan_pairs(b, h->stream);
+    h->pair_lookback_clean = false;
 }
 
 void mark(nlh_handle *h, int k)
@@ -650,6 +663,10 @@ void finish_status(nlh_handle *h)
 {
     HIPC(hipMemcpyAsync(h->status_host, h->status.p, sizeof(nlh::Status), hipMemcpyDeviceToHost, h->stream));
     HIPC(hipStreamSynchronize(h->stream));
+    if (h->status_host->scan_error) {
+        h->built = false;
+        throw Error(NLH_EDEVICE, "internal: a scan block waited too long for its predecessors");
+    }
 }
 
 void check_positions(nlh_handle *h)
@@ -710,7 +727,7 @@ void materialize(nlh_handle *h, uint32_t flags)
     mark(h, 1);
     nlh::launch_count(b, want_R, h->wide, h->stream);
     mark(h, 2);
-    nlh::launch_scan_pairs(b, h->stream);
+    pair_scan(h, b);
     mark(h, 3);
     nlh::launch_fill(b, want_R, h->wide, h->stream);
     mark(h, 4);
@@ -729,7 +746,7 @@ void materialize(nlh_handle *h, uint32_t flags)
         check_pair_buffers(h, want_R);
         b = h->bufs();
         nlh::launch_count(b, want_R, h->wide, h->stream);
-        nlh::launch_scan_pairs(b, h->stream);   // fill tiles over the grown capacity
+        pair_scan(h, b);   // fill tiles over the grown capacity
         nlh::launch_fill(b, want_R, h->wide, h->stream);
         HIPC(hipGetLastError());
         finish_status(h);
@@ -774,7 +791,7 @@ void capture_graph(nlh_handle *h, bool want_R)
         // kernel launches only between begin and end (nothing that takes the lock)
         std::unique_lock<std::shared_mutex> lk(g_capture_mutex);
         HIPC(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-        nlh::launch_clear_status(b.status, h->stream);
+        nlh::launch_clear_status(b, h->stream);
         nlh::launch_build(b, h->stream);
         nlh::launch_count(b, want_R, h->wide, h->stream);
         nlh::launch_scan_pairs(b, h->stream);
@@ -805,6 +822,7 @@ bool rebuild_graph(nlh_handle *h, bool want_R)
     if (!h->graph_exec || h->graph_want_R != want_R || std::memcmp(&b, &h->graph_bufs, sizeof b) != 0)
         capture_graph(h, want_R);
     HIPC(hipGraphLaunch(h->graph_exec, h->stream));
+    h->pair_lookback_clean = false;   // the replay's pair scan used them
     ++h->graph_launches;
     h->built = true;
     h->materialized = false;
@@ -965,7 +983,7 @@ int nlh_destroy(nlh_handle *h)
         });
         DevBuf *bufs[] = {&h->X,        &h->tables,    &h->key,    &h->slot,
                           &h->cell_count, &h->cell_start, &h->tmp,      &h->rec,          &h->wbig,   &h->perm,
-                          &h->cell_id,  &h->cell_offsets, &h->pair_count, &h->inv_perm, &h->slots, &h->slotsR, &h->pair_off,   &h->tiles, &h->partials,
+                          &h->cell_id,  &h->cell_offsets, &h->pair_count, &h->inv_perm, &h->slots, &h->slotsR, &h->pair_off,   &h->tiles, &h->lookback,
                           &h->sys_pair_base, &h->first, &h->i,          &h->j,            &h->S,      &h->R,
                           &h->status,   &h->Xsorted,   &h->nb_j,       &h->nb_R,         &h->nb_S,   &h->nb_count};
         for (DevBuf *b : bufs) b->release();

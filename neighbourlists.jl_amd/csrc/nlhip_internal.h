@@ -21,6 +21,13 @@ constexpr int kScanItems = 8;             // elements per thread in a scan tile
 constexpr int kScanTile = kScanThreads * kScanItems;
 constexpr int kMinFillPairs = 128;        // smallest fill tile (pairs per fill wave)
 
+// Single-pass scan descriptors (a ticket plus one per tile of kScanTile
+// items): the cell scan's, then the pair scan's.
+inline int64_t scan_tiles(int64_t n) { return n > 0 ? (n + kScanTile - 1) / kScanTile : 1; }
+constexpr int64_t kLookbackTiles = 64;   // single-pass scans up to this many tiles
+inline int64_t lookback_cells(int64_t M) { return scan_tiles(M) + 1; }
+inline int64_t lookback_entries(int64_t M, int64_t nat) { return lookback_cells(M) + scan_tiles(nat) + 1; }
+
 // log2 of the fill tile (pairs per fill wave: 128, 256 or 512)
 inline int fill_tile_shift(int fill_pairs) { return fill_pairs == 128 ? 7 : (fill_pairs == 512 ? 9 : 8); }
 
@@ -84,7 +91,8 @@ __host__ __device__ inline void unpack_wind(uint32_t w, int32_t &wx, int32_t &wy
 // Device-side status block, copied to pinned host memory once per build.
 struct Status {
     int64_t npairs;          // total pairs over the batch
-    int64_t reserved;
+    int32_t scan_error;      // a scan block's look-back exceeded its bound (internal error)
+    int32_t spare;
     int32_t overflow;        // pair capacity exceeded: outputs not written
     int32_t bad_position;    // non-finite / out-of-range position seen
     int32_t wide;            // a cell shift does not fit a packed slot entry
@@ -193,7 +201,7 @@ struct PipelineBufs {
     int64_t *pair_off;          // nat + 1
     int32_t *tile_atom;         // 2 tile_cap: atom owning pair t * fill_pairs, its pairs before it
     int64_t tile_cap;
-    int64_t *scan_partials;     // scan block sums
+    uint64_t *lookback;         // single-pass scan descriptors: cells (ticket + tiles), then pairs
     int64_t *sys_pair_base;     // nsys + 1 (device)
     void *first_out;            // nat + nsys, TI, 1-based per system
     void *i_out, *j_out, *S_out;
@@ -206,7 +214,8 @@ struct PipelineBufs {
     int count_block;         // threads per count-pass brick workgroup (256 or 512)
 };
 
-void launch_clear_status(Status *status, hipStream_t st);
+void launch_clear_status(const PipelineBufs &b, hipStream_t st);
+void launch_clear_pair_lookback(const PipelineBufs &b, hipStream_t st);
 void launch_build(const PipelineBufs &b, hipStream_t st);
 void launch_count(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st);
 void launch_scan_pairs(const PipelineBufs &b, hipStream_t st);

@@ -253,3 +253,25 @@ def test_fill_tiles_over_many_empty_atoms(fill_pairs):
     nl = nlhip.neighbour_list(Xc, 5.0, Cg, (True, True, True), want_R=True, handle=h)
     ref = oracle.neighbour_list(Xc, 5.0, Cg, (True, True, True), want_R=True)
     _same(nl, ref)
+
+
+def test_scan_paths():
+    """Both scan forms: a single pass up to 64 tiles of 2048 items, reduce ->
+    tile sums -> downsweep beyond.  300k atoms: 147 tiles of pair counts
+    (three kernels), 23 of cells (single pass); a sparse 100k-atom box:
+    over 3 million cells (three kernels), 49 tiles of pair counts (single
+    pass)."""
+    s = workloads.random_cubic(300_000, density=0.05, seed=95)
+    print("scan paths: 300k cube", flush=True)
+    nl = nlhip.neighbour_list(s.X, 5.0, s.cell, s.pbc)
+    ref = oracle.neighbour_list(s.X, 5.0, s.cell, s.pbc)
+    _same(nl, ref, R=False)
+    cl = nlhip.build_cell_list(s.X, 5.0, s.cell, s.pbc)
+    for k in ("perm", "cell_id", "cell_offsets"):
+        assert np.array_equal(getattr(cl, k), getattr(ref, k)), k
+    s = workloads.random_cubic(100_000, density=0.002, seed=96, cutoff=2.5)
+    print("scan paths: sparse 100k, 3.2M cells", flush=True)
+    _, ref = _check(s.X, 2.5, s.cell, (True, False, True))
+    cl = nlhip.build_cell_list(s.X, 2.5, s.cell, (True, False, True))
+    for k in ("perm", "cell_id", "cell_offsets"):
+        assert np.array_equal(getattr(cl, k), getattr(ref, k)), k
