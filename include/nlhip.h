@@ -46,7 +46,8 @@ typedef enum {
 #define NLH_X_DEVICE 0x1u  /* X (and X_batch) is a device pointer             */
 #define NLH_WANT_R 0x2u    /* also write R = X[j] - X[i] + C' S per pair      */
 #define NLH_NO_SYNC 0x4u   /* do not wait for the stream (nlh_synchronize)    */
-#define NLH_NO_GRAPH 0x8u  /* nlh_batch_rebuild: run the stages one by one    */
+#define NLH_NO_GRAPH 0x8u  /* nlh_batch_rebuild, nlh_neighbour_list: run the
+                              stages one by one (no graph replay)            */
 
 /* Geometry of one system: the outputs of analyze_cell plus the neighbour
  * reach and cutoff^2 (src/cell_list.jl:152-170, 655-659;
@@ -115,7 +116,11 @@ int nlh_build_cell_list(nlh_handle *h, const double *X, int64_t nat, const nlh_g
 int nlh_materialize(nlh_handle *h, uint32_t flags, int64_t *npairs);
 
 /* neighbour_list(X, cutoff, cell, pbc; lazy=false) in one call
- * (src/cell_list.jl:897-916): build + materialise with a single wait. */
+ * (src/cell_list.jl:897-916): build + materialise with a single wait.  A
+ * call that repeats the previous call's shape on this handle (same atom
+ * count, cell grid, positions buffer, capacities and flags; positions and
+ * geometry values may differ) replays the pipeline captured as a HIP graph,
+ * with identical results; NLH_NO_GRAPH runs the stages one by one. */
 int nlh_neighbour_list(nlh_handle *h, const double *X, int64_t nat, const nlh_geometry *geom, uint32_t flags,
                        int64_t *npairs);
 
@@ -193,6 +198,11 @@ int nlh_autotune(nlh_handle *h, const double *X_batch, uint32_t flags, int32_t r
  * scripts/benchmark.jl:124-151). */
 int nlh_set_profiling(nlh_handle *h, int enable);
 int nlh_phase_times(nlh_handle *h, float ms[5]);
+
+/* Number of builds on this handle that replayed the captured HIP graph
+ * (nlh_batch_rebuild, repeated nlh_neighbour_list).  No reference
+ * counterpart: a diagnostic of the replay path. */
+int nlh_graph_launches(nlh_handle *h, int64_t *n);
 
 /* Device memory of the library's own HIP runtime, for callers that keep
  * positions resident without another GPU runtime (no reference

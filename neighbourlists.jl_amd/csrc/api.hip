@@ -118,7 +118,10 @@ struct nlh_handle {
     int64_t *atom_base_d = nullptr;
     int32_t *cell_base_d = nullptr, *brick_base_d = nullptr;
     std::vector<char> tables_host;        // bytes of the last upload
-    nlh::Status *status_host = nullptr;   // pinned
+    nlh::Status *status_host = nullptr;   // pinned, mapped: the fill kernel writes it
+    nlh::Status *status_host_dev = nullptr;   // its device address
+    bool status_pub_pending = false;      // the last launched fill publishes the status
+    bool graph_publishes = false;         // the captured pipeline ends with a fill
     void *staging = nullptr;              // pinned upload staging for system tables
     size_t staging_cap = 0;
     void *xstage = nullptr;               // pinned upload staging for host positions
@@ -154,6 +157,9 @@ struct nlh_handle {
     hipGraph_t graph = nullptr;
     bool graph_want_R = false;
     nlh::PipelineBufs graph_bufs;
+    bool last_single = false;            // the previous call was a full single build ...
+    uint32_t last_flags = 0;             // ... with these flags
+    nlh::PipelineBufs last_bufs;         // ... on these buffers and sizes
     int64_t graph_launches = 0;
     // launch configuration (nlh_set_tuning / nlh_autotune)
     nlh_tuning tune = {768, 256, 256, 0};
@@ -202,6 +208,7 @@ struct nlh_handle {
         b.R_out = R.as<double>();
         b.pair_capacity = pair_capacity;
         b.status = status.as<nlh::Status>();
+        b.status_pub = status_host_dev;
         b.ti_bytes = ti_bytes;
         b.fill_block = tune.fill_block;
         b.fill_pairs = tune.fill_pairs;
@@ -558,6 +565,7 @@ void reset_status(nlh_handle *h)
 {
     nlh::launch_clear_status(h->bufs(), h->stream);
     h->pair_lookback_clean = true;
+    h->status_pub_pending = false;
 }
 
 // The pair scan, on descriptors that start from zero: cleared with the
@@ -661,8 +669,12 @@ void ensure_pair_buffers(nlh_handle *h)
 
 void finish_status(nlh_handle *h)
 {
-    HIPC(hipMemcpyAsync(h->status_host, h->status.p, sizeof(nlh::Status), hipMemcpyDeviceToHost, h->stream));
+    // a fill publishes the status itself (the pair stages end with it);
+    // otherwise it is copied back
+    if (!h->status_pub_pending)
+        HIPC(hipMemcpyAsync(h->status_host, h->status.p, sizeof(nlh::Status), hipMemcpyDeviceToHost, h->stream));
     HIPC(hipStreamSynchronize(h->stream));
+    h->status_pub_pending = false;
     if (h->status_host->scan_error) {
         h->built = false;
         throw Error(NLH_EDEVICE, "internal: a scan block waited too long for its predecessors");
@@ -729,7 +741,7 @@ void materialize(nlh_handle *h, uint32_t flags)
     mark(h, 2);
     pair_scan(h, b);
     mark(h, 3);
-    nlh::launch_fill(b, want_R, h->wide, h->stream);
+    h->status_pub_pending = nlh::launch_fill(b, want_R, h->wide, h->stream);
     mark(h, 4);
     h->ev_pairs = h->profile;
     HIPC(hipGetLastError());
@@ -747,7 +759,7 @@ void materialize(nlh_handle *h, uint32_t flags)
         b = h->bufs();
         nlh::launch_count(b, want_R, h->wide, h->stream);
         pair_scan(h, b);   // fill tiles over the grown capacity
-        nlh::launch_fill(b, want_R, h->wide, h->stream);
+        h->status_pub_pending = nlh::launch_fill(b, want_R, h->wide, h->stream);
         HIPC(hipGetLastError());
         finish_status(h);
         if (h->status_host->npairs > h->pair_capacity || h->status_host->max_count > h->slot_cap)
@@ -795,7 +807,7 @@ void capture_graph(nlh_handle *h, bool want_R)
         nlh::launch_build(b, h->stream);
         nlh::launch_count(b, want_R, h->wide, h->stream);
         nlh::launch_scan_pairs(b, h->stream);
-        nlh::launch_fill(b, want_R, h->wide, h->stream);
+        h->graph_publishes = nlh::launch_fill(b, want_R, h->wide, h->stream);
         le = hipGetLastError();
         ee = hipStreamEndCapture(h->stream, &g);
     }
@@ -823,6 +835,7 @@ bool rebuild_graph(nlh_handle *h, bool want_R)
         capture_graph(h, want_R);
     HIPC(hipGraphLaunch(h->graph_exec, h->stream));
     h->pair_lookback_clean = false;   // the replay's pair scan used them
+    h->status_pub_pending = h->graph_publishes;
     ++h->graph_launches;
     h->built = true;
     h->materialized = false;
@@ -960,8 +973,10 @@ int nlh_create(nlh_handle **out)
         try {
             HIPC(hipSetDevice(h->device));
             HIPD(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-            HIPD(hipHostMalloc((void **)&h->status_host, sizeof(nlh::Status), hipHostMallocDefault));
+            HIPD(hipHostMalloc((void **)&h->status_host, sizeof(nlh::Status),
+                               hipHostMallocMapped | hipHostMallocCoherent));
             std::memset(h->status_host, 0, sizeof(nlh::Status));
+            HIPD(hipHostGetDevicePointer((void **)&h->status_host_dev, h->status_host, 0));
         } catch (...) {
             delete h;
             throw;
@@ -1030,6 +1045,7 @@ int nlh_build_cell_list(nlh_handle *h, const double *X, int64_t nat, const nlh_g
         if (!h || !geom) throw Error(NLH_EINVAL, "NULL argument");
         if (nat < 0) throw Error(NLH_EINVAL, "nat < 0");
         HIPC(hipSetDevice(h->device));
+        h->last_single = false;   // not a single full build: no repeat to replay
         h->built = false;
         drop_graph(h);
         const int64_t offs[2] = {0, nat};
@@ -1062,13 +1078,24 @@ int nlh_neighbour_list(nlh_handle *h, const double *X, int64_t nat, const nlh_ge
         if (nat < 0) throw Error(NLH_EINVAL, "nat < 0");
         HIPC(hipSetDevice(h->device));
         h->built = false;
-        drop_graph(h);
         const int64_t offs[2] = {0, nat};
         setup_batch(h, 1, offs, geom);
         upload_positions(h, X, flags);
-        reset_status(h);
-        build_cells(h);
-        materialize(h, flags);
+        // a repeat of the previous call's shape replays the captured pipeline
+        // (captured on the first repeat); otherwise, or when a replay
+        // outgrew a capacity, the stages run one by one
+        const nlh::PipelineBufs b = h->bufs();
+        const bool repeat = h->last_single && h->last_flags == flags && std::memcmp(&b, &h->last_bufs, sizeof b) == 0;
+        const bool graph = repeat && !(flags & NLH_NO_GRAPH) && !h->profile && h->materialized && h->nat > 0;
+        h->last_single = false;
+        if (!(graph && rebuild_graph(h, (flags & NLH_WANT_R) != 0))) {
+            reset_status(h);
+            build_cells(h);
+            materialize(h, flags);
+        }
+        h->last_single = true;
+        h->last_flags = flags;
+        h->last_bufs = h->bufs();
         if (npairs) *npairs = h->npairs;
     });
 }
@@ -1206,6 +1233,14 @@ int nlh_set_profiling(nlh_handle *h, int enable)
     });
 }
 
+int nlh_graph_launches(nlh_handle *h, int64_t *n)
+{
+    return guarded([&] {
+        if (!h || !n) throw Error(NLH_EINVAL, "NULL argument");
+        *n = h->graph_launches;
+    });
+}
+
 int nlh_phase_times(nlh_handle *h, float ms[5])
 {
     return guarded([&] {
@@ -1228,6 +1263,7 @@ int nlh_batch_neighbour_list(nlh_handle *h, int32_t nsys, const double *X_batch,
     return guarded([&] {
         if (!h || !atom_offsets || !geoms) throw Error(NLH_EINVAL, "NULL argument");
         HIPC(hipSetDevice(h->device));
+        h->last_single = false;   // not a single full build: no repeat to replay
         h->built = false;
         drop_graph(h);
         setup_batch(h, nsys, atom_offsets, geoms);
@@ -1254,6 +1290,7 @@ int nlh_batch_rebuild(nlh_handle *h, const double *X_batch, uint32_t flags, int6
         if (!h) throw Error(NLH_EINVAL, "handle is NULL");
         if (h->sys_host.empty()) throw Error(NLH_ESTATE, "no previous batch to rebuild");
         HIPC(hipSetDevice(h->device));
+        h->last_single = false;   // not a single full build: no repeat to replay
         upload_positions(h, X_batch, flags);
         const bool want_R = (flags & NLH_WANT_R) != 0;
         const bool graph = !(flags & NLH_NO_GRAPH) && !h->profile && h->materialized && h->nat > 0;

@@ -208,6 +208,7 @@ struct PipelineBufs {
     double *R_out;
     int64_t pair_capacity;
     Status *status;
+    Status *status_pub;         // device address of the handle's pinned host copy of the status
     int ti_bytes;
     int fill_block;          // threads per fill workgroup (multiple of 64)
     int fill_pairs;          // output pairs per fill wave (128, 256 or 512)
@@ -219,7 +220,9 @@ void launch_clear_pair_lookback(const PipelineBufs &b, hipStream_t st);
 void launch_build(const PipelineBufs &b, hipStream_t st);
 void launch_count(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st);
 void launch_scan_pairs(const PipelineBufs &b, hipStream_t st);
-void launch_fill(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st);
+// Returns whether a fill kernel was launched (it then also publishes the
+// status to status_pub).
+bool launch_fill(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st);
 void launch_sorted_positions(const PipelineBufs &b, double *Xs, hipStream_t st);
 
 }  // namespace nlh

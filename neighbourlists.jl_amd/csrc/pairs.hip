@@ -868,9 +868,9 @@ template <bool WANT_R, bool WIDE, typename TI, int TW>
 __global__ __launch_bounds__(1024) void fill_kernel(
     const int64_t *__restrict__ sys_atom_base, int nsys, int64_t nat, const int32_t *__restrict__ pair_count,
     const int64_t *__restrict__ pair_off, const int32_t *__restrict__ tile_atom, int64_t tile_cap,
-    const void *__restrict__ slots, const double *__restrict__ slotsR, const Status *__restrict__ status, int32_t cap,
-    int64_t pair_capacity, TI *__restrict__ i_out, TI *__restrict__ j_out, TI *__restrict__ S_out,
-    double *__restrict__ R_out)
+    const void *__restrict__ slots, const double *__restrict__ slotsR, const Status *__restrict__ status,
+    Status *__restrict__ status_pub, int32_t cap, int64_t pair_capacity, TI *__restrict__ i_out,
+    TI *__restrict__ j_out, TI *__restrict__ S_out, double *__restrict__ R_out)
 {
     constexpr int TP = TW * kWave;
     constexpr int32_t kBeyond = 1 << 30;   // window position past every pair of the tile
@@ -879,6 +879,13 @@ __global__ __launch_bounds__(1024) void fill_kernel(
     // the output (an XCD-contiguous block order measured much slower)
     const int64_t w = (int64_t)blockIdx.x * (blockDim.x / kWave) + sgpr((int)(threadIdx.x / kWave));   // wave-uniform
     const int64_t Q0 = w * TP;
+    if (blockIdx.x == 0 && threadIdx.x < sizeof(Status) / 4) {
+        // the status is final once the fill runs: publish it to the host's
+        // pinned copy (system-scope stores), so the host only has to wait
+        const int32_t v = reinterpret_cast<const int32_t *>(status)[threadIdx.x];
+        __hip_atomic_store(reinterpret_cast<int32_t *>(status_pub) + threadIdx.x, v, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
+    }
     const Status st = *status;
     // the atoms owning this tile's first pair and the next tile's, read with
     // the status (indexes clamped to the table, which holds at least two
@@ -1013,43 +1020,38 @@ void launch_count(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st)
 }
 
 template <bool R, bool W, typename TI, int TW>
-static void fill_launch(const PipelineBufs &b, hipStream_t st)
+static bool fill_launch(const PipelineBufs &b, hipStream_t st)
 {
     // one wave per tile of the pair capacity (waves past the pairs return)
     const int bs = b.fill_block;
     const int64_t waves = (b.pair_capacity + TW * kWave - 1) / (TW * kWave);
-    if (waves <= 0) return;
+    if (waves <= 0) return false;
     const unsigned nblk = (unsigned)((waves * kWave + bs - 1) / bs);
     hipLaunchKernelGGL((fill_kernel<R, W, TI, TW>), dim3(nblk), dim3(bs), 0, st, b.sys_atom_base, b.nsys, b.nat,
-                       b.pair_count, b.pair_off, b.tile_atom, b.tile_cap, b.slots, b.slotsR, b.status, b.slot_cap,
-                       b.pair_capacity, (TI *)b.i_out, (TI *)b.j_out, (TI *)b.S_out, b.R_out);
+                       b.pair_count, b.pair_off, b.tile_atom, b.tile_cap, b.slots, b.slotsR, b.status, b.status_pub,
+                       b.slot_cap, b.pair_capacity, (TI *)b.i_out, (TI *)b.j_out, (TI *)b.S_out, b.R_out);
+    return true;
 }
 
 template <bool R, bool W, typename TI>
-static void fill_tiles(const PipelineBufs &b, hipStream_t st)
+static bool fill_tiles(const PipelineBufs &b, hipStream_t st)
 {
-    if (b.fill_pairs == 128) fill_launch<R, W, TI, 2>(b, st);
-    else if (b.fill_pairs == 512) fill_launch<R, W, TI, 8>(b, st);
-    else fill_launch<R, W, TI, 4>(b, st);
+    if (b.fill_pairs == 128) return fill_launch<R, W, TI, 2>(b, st);
+    if (b.fill_pairs == 512) return fill_launch<R, W, TI, 8>(b, st);
+    return fill_launch<R, W, TI, 4>(b, st);
 }
 
 template <typename TI>
-static void fill_dispatch(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st)
+static bool fill_dispatch(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st)
 {
-    if (want_R) {
-        if (wide) fill_tiles<true, true, TI>(b, st);
-        else fill_tiles<true, false, TI>(b, st);
-    } else {
-        if (wide) fill_tiles<false, true, TI>(b, st);
-        else fill_tiles<false, false, TI>(b, st);
-    }
+    if (want_R) return wide ? fill_tiles<true, true, TI>(b, st) : fill_tiles<true, false, TI>(b, st);
+    return wide ? fill_tiles<false, true, TI>(b, st) : fill_tiles<false, false, TI>(b, st);
 }
 
-void launch_fill(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st)
+bool launch_fill(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st)
 {
-    if (b.nat == 0) return;
-    if (b.ti_bytes == 8) fill_dispatch<int64_t>(b, want_R, wide, st);
-    else fill_dispatch<int32_t>(b, want_R, wide, st);
+    if (b.nat == 0) return false;
+    return b.ti_bytes == 8 ? fill_dispatch<int64_t>(b, want_R, wide, st) : fill_dispatch<int32_t>(b, want_R, wide, st);
 }
 
 }  // namespace nlh

@@ -110,7 +110,7 @@ EXPORTS = [
     "nlh_pairlist_device", "nlh_celllist_device", "nlh_copy_pairlist", "nlh_copy_celllist",
     "nlh_neighbours", "nlh_batch_neighbour_list", "nlh_batch_pair_offsets", "nlh_batch_rebuild",
     "nlh_set_tuning", "nlh_get_tuning", "nlh_autotune",
-    "nlh_set_profiling", "nlh_phase_times", "nlh_device_malloc", "nlh_device_free", "nlh_memcpy", "nlh_device_synchronize",
+    "nlh_set_profiling", "nlh_phase_times", "nlh_graph_launches", "nlh_device_malloc", "nlh_device_free", "nlh_memcpy", "nlh_device_synchronize",
     "nlh_version",
 ]
 
@@ -141,6 +141,7 @@ def _declare(L):
     L.nlh_autotune.argtypes = [hp, vp, C.c_uint32, C.c_int32, C.POINTER(Tuning)]
     L.nlh_set_profiling.argtypes = [hp, C.c_int]
     L.nlh_phase_times.argtypes = [hp, C.POINTER(C.c_float)]
+    L.nlh_graph_launches.argtypes = [hp, C.POINTER(C.c_int64)]
     L.nlh_device_malloc.argtypes = [C.c_uint64, C.POINTER(C.c_void_p)]
     L.nlh_device_free.argtypes = [C.c_void_p]
     L.nlh_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]
@@ -282,6 +283,12 @@ class Handle:
 
     def set_profiling(self, enable: bool = True):
         check(lib().nlh_set_profiling(self._h, 1 if enable else 0), "set_profiling")
+
+    def graph_launches(self) -> int:
+        """Builds on this handle that replayed the captured HIP graph."""
+        n = C.c_int64()
+        check(lib().nlh_graph_launches(self._h, C.byref(n)), "graph_launches")
+        return int(n.value)
 
     def phase_times(self) -> dict:
         """ms of the last build: cells, count, scan, fill, total (HIP events)."""

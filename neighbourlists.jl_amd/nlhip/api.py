@@ -227,9 +227,10 @@ def _collect_pairs(h: Handle, g: Geometry, X, P: int, want_R: bool, dev) -> Pair
 
 
 def neighbour_list(X, cutoff, cell, pbc, *, lazy: bool = False, int_type=Int32, want_R: bool = False,
-                   handle: Handle | None = None):
+                   handle: Handle | None = None, no_graph: bool = False):
     """Unified entry point (cell_list.jl:897-916): a PairList, or with
-    ``lazy=True`` a SortedCellList."""
+    ``lazy=True`` a SortedCellList.  Repeated calls of one shape on a handle
+    replay a captured HIP graph unless ``no_graph``."""
     if lazy:
         return build_cell_list(X, cutoff, cell, pbc, int_type=int_type, handle=handle)
     ptr, nat, flags, keep, dev = _prep_positions(X)
@@ -238,6 +239,8 @@ def neighbour_list(X, cutoff, cell, pbc, *, lazy: bool = False, int_type=Int32, 
     n = C.c_int64()
     if want_R:
         flags |= NLH_WANT_R
+    if no_graph:
+        flags |= _lib.NLH_NO_GRAPH
     check(lib().nlh_neighbour_list(h.ptr, ptr, nat, C.byref(g), flags, C.byref(n)), "neighbour_list")
     return _collect_pairs(h, g, keep if dev is None else dev, n.value, want_R, dev)
 

@@ -275,3 +275,29 @@ def test_scan_paths():
     cl = nlhip.build_cell_list(s.X, 2.5, s.cell, (True, False, True))
     for k in ("perm", "cell_id", "cell_offsets"):
         assert np.array_equal(getattr(cl, k), getattr(ref, k)), k
+
+
+def test_repeated_builds_replay_the_pipeline():
+    """Repeated single builds of one shape on a handle replay the captured
+    pipeline (from the first repeat on); every result equals the oracle:
+    new positions each call, a cell scaled within the same cell grid, another
+    shape built in between, and a call that asks for the stages one by one."""
+    h = nlhip._lib.Handle()
+    s = workloads.random_cubic(3000, seed=97)            # L/rc = 7.83: 7 cells per axis
+    other = workloads.random_cubic(1000, seed=98)
+    for rep in range(6):
+        X = s.X + 0.05 * workloads.normal(300 + rep, s.X.size).reshape(s.X.shape)
+        C = s.cell * (1.0 + 0.001 * rep)
+        print(f"repeat {rep} graph launches {h.graph_launches()}", flush=True)
+        nl = nlhip.neighbour_list(X, 5.0, C, s.pbc, want_R=True, handle=h)
+        _same(nl, oracle.neighbour_list(X, 5.0, C, s.pbc, want_R=True))
+        if rep == 3:
+            nl = nlhip.neighbour_list(other.X, 5.0, other.cell, other.pbc, want_R=True, handle=h)
+            _same(nl, oracle.neighbour_list(other.X, 5.0, other.cell, other.pbc, want_R=True))
+    # calls 2, 3, 4 (repeats) and 6 (the second repeat after the other shape)
+    assert h.graph_launches() == 4
+    n0 = h.graph_launches()
+    X = s.X + 0.05 * workloads.normal(400, s.X.size).reshape(s.X.shape)
+    nl = nlhip.neighbour_list(X, 5.0, s.cell, s.pbc, want_R=True, handle=h, no_graph=True)
+    _same(nl, oracle.neighbour_list(X, 5.0, s.cell, s.pbc, want_R=True))
+    assert h.graph_launches() == n0
