@@ -649,6 +649,48 @@ __device__ __forceinline__ void test_home_atom(int nch, const double *cx, const 
     }
 }
 
+// Two consecutive home atoms of a 4-byte brick (w_i = 0 for both, so the
+// same shifts) against the same chunks: both distance tests of a chunk come
+// before both emissions, two independent chains the scheduler interleaves.
+// Atom 1's self candidate follows atom 0's.
+template <bool WANT_R, bool WIDE>
+__device__ __forceinline__ void test_home_pair(int nch, const double *cx, const double *cy, const double *cz,
+                                               const double *sx, const double *sy, const double *sz, const bool *csh,
+                                               const uint64_t *ent, double h0x, double h0y, double h0z, double h1x,
+                                               double h1y, double h1z, double rc2, int ks0, void *__restrict__ row0,
+                                               void *__restrict__ row1, double *__restrict__ rrow0,
+                                               double *__restrict__ rrow1, uint32_t cap, Status *__restrict__ status,
+                                               int32_t &ca0, int32_t &ca1)
+{
+#pragma unroll
+    for (int g = 0; g < kBrickChunks; ++g) {
+        if (g < nch) {
+            // R = xj - xi + C' * S ; r^2 = dot(R, R)  (gpu_kernels.jl:89-92)
+            double R0x = cx[g] - h0x, R0y = cy[g] - h0y, R0z = cz[g] - h0z;
+            double R1x = cx[g] - h1x, R1y = cy[g] - h1y, R1z = cz[g] - h1z;
+            if (WANT_R || csh[g]) {
+                R0x = R0x + sx[g];
+                R0y = R0y + sy[g];
+                R0z = R0z + sz[g];
+                R1x = R1x + sx[g];
+                R1y = R1y + sy[g];
+                R1z = R1z + sz[g];
+            }
+            const bool in0 = (R0x * R0x + R0y * R0y) + R0z * R0z < rc2;
+            const bool in1 = (R1x * R1x + R1y * R1y) + R1z * R1z < rc2;
+            const int d0 = ks0 - g * kWave, d1 = d0 + 1;
+            const uint64_t self0 = (unsigned)d0 < (unsigned)kWave ? ~(1ull << d0) : ~0ull;
+            const uint64_t self1 = (unsigned)d1 < (unsigned)kWave ? ~(1ull << d1) : ~0ull;
+            const uint64_t m0 = __builtin_amdgcn_ballot_w64(in0) & self0;
+            const uint64_t m1 = __builtin_amdgcn_ballot_w64(in1) & self1;
+            emit_row<true, true, WANT_R, WIDE>(row0, rrow0, cap, status, in0 && lane_id() != d0, m0, ca0, ent[g], 0, 0,
+                                               0, R0x, R0y, R0z);
+            emit_row<true, true, WANT_R, WIDE>(row1, rrow1, cap, status, in1 && lane_id() != d1, m1, ca1, ent[g], 0, 0,
+                                               0, R1x, R1y, R1z);
+        }
+    }
+}
+
 // Brick pass of the count: home atoms stream through the wave as scalars
 // (read out of one-per-lane registers), candidates sit one per lane in
 // registers, chunk by chunk; each (home atom, chunk) is one FP64 distance
@@ -723,7 +765,43 @@ __global__ __launch_bounds__(BT) void count_kernel(
                 uint32_t wcur = 0xffffffffu;   // home winding the shifts were computed for
                 int32_t wix = 0, wiy = 0, wiz = 0;
                 bool csh[kBrickChunks];        // wave-uniform: some lane of the chunk has a nonzero shift
-                for (int a = 0; a < nh; ++a) {
+                int a = 0;
+                if (narrow && nh >= 2) {
+                    // every home atom of a 4-byte brick has w_i = 0: the shifts of
+                    // the candidates are their own cell shifts
+#pragma unroll
+                    for (int g = 0; g < kBrickChunks; ++g) {
+                        sx[g] = sy[g] = sz[g] = 0.0;
+                        csh[g] = false;
+                        if (g < nch) {
+                            int32_t cx_, cy_, cz_;
+                            unpack_wind((uint32_t)(ent[g] >> 32), cx_, cy_, cz_);
+                            csh[g] = __builtin_amdgcn_ballot_w64((cx_ | cy_ | cz_) != 0) != 0;
+                            if (csh[g]) cell_shift(Csh, cx_, cy_, cz_, sx[g], sy[g], sz[g]);
+                        }
+                    }
+                    wcur = kPackZero;
+                    for (; a + 1 < nh; a += 2) {
+                        const double h0x = readlane_f64(hx_l, a), h0y = readlane_f64(hy_l, a),
+                                     h0z = readlane_f64(hz_l, a);
+                        const double h1x = readlane_f64(hx_l, a + 1), h1y = readlane_f64(hy_l, a + 1),
+                                     h1z = readlane_f64(hz_l, a + 1);
+                        int32_t ca0 = __builtin_amdgcn_readlane(cnt_l, a), ca1 = __builtin_amdgcn_readlane(cnt_l, a + 1);
+                        const int64_t srow0 = (sd.atom_base + __builtin_amdgcn_readlane(hj_l, a)) * (int64_t)cap;
+                        const int64_t srow1 = (sd.atom_base + __builtin_amdgcn_readlane(hj_l, a + 1)) * (int64_t)cap;
+                        void *row0 = WIDE ? (void *)(reinterpret_cast<int4 *>(slots) + srow0)
+                                          : (void *)(reinterpret_cast<uint64_t *>(slots) + srow0);
+                        void *row1 = WIDE ? (void *)(reinterpret_cast<int4 *>(slots) + srow1)
+                                          : (void *)(reinterpret_cast<uint64_t *>(slots) + srow1);
+                        double *rrow0 = WANT_R ? slotsR + 3 * srow0 : nullptr;
+                        double *rrow1 = WANT_R ? slotsR + 3 * srow1 : nullptr;
+                        test_home_pair<WANT_R, WIDE>(nch, cx, cy, cz, sx, sy, sz, csh, ent, h0x, h0y, h0z, h1x, h1y,
+                                                     h1z, rc2, kself + a - k0, row0, row1, rrow0, rrow1,
+                                                     (uint32_t)cap, status, ca0, ca1);
+                        cnt_l = lane == a ? ca0 : (lane == a + 1 ? ca1 : cnt_l);
+                    }
+                }
+                for (; a < nh; ++a) {
                     const double hx = readlane_f64(hx_l, a), hy = readlane_f64(hy_l, a), hz = readlane_f64(hz_l, a);
                     const uint32_t ht = (uint32_t)__builtin_amdgcn_readlane((int)ht_l, a);
                     if (ht != wcur) {   // first home atom, or a new home winding
