@@ -110,7 +110,7 @@ struct nlh_handle {
     int device = 0;
     hipStream_t stream = nullptr;
     DevBuf X, tables, key, slot, cell_count, cell_start, tmp, rec, wbig, perm, cell_id,
-        cell_offsets, pair_count, inv_perm, slots, slotsR, pair_off, tiles, lookback, sys_pair_base, first, i, j, S, R, status, Xsorted, nb_j, nb_R,
+        cell_offsets, pair_count, inv_perm, slots, slotsR, pair_off, tiles, lookback, sysmap, sys_pair_base, first, i, j, S, R, status, Xsorted, nb_j, nb_R,
         nb_S, nb_count;
     // system tables, one device buffer: SysDev[nsys], atom bases int64[nsys + 1],
     // cell bases int32[nsys + 1], brick bases int32[nsys + 1]
@@ -200,6 +200,7 @@ struct nlh_handle {
         b.tile_atom = tiles.as<int32_t>();
         b.tile_cap = (int64_t)(tiles.cap / 8);
         b.lookback = lookback.as<uint64_t>();
+        b.atom_sys = nsys > 1 ? sysmap.as<int32_t>() : nullptr;
         b.sys_pair_base = sys_pair_base.as<int64_t>();
         b.first_out = first.p;
         b.i_out = i.p;
@@ -517,6 +518,7 @@ void setup_batch(nlh_handle *h, int32_t nsys, const int64_t *atom_offsets, const
     h->cell_id.ensure(h->stream, n1 * ti);
     h->cell_offsets.ensure(h->stream, (M1 + nsys) * ti);
     h->lookback.ensure(h->stream, (size_t)nlh::lookback_entries((int64_t)M1, (int64_t)n1) * sizeof(uint64_t));
+    if (nsys > 1) h->sysmap.ensure(h->stream, (n1 + M1) * 4);   // system of each atom, then of each cell
     h->status.ensure(h->stream, sizeof(nlh::Status));
 
     upload_tables(h);
@@ -998,7 +1000,7 @@ int nlh_destroy(nlh_handle *h)
         });
         DevBuf *bufs[] = {&h->X,        &h->tables,    &h->key,    &h->slot,
                           &h->cell_count, &h->cell_start, &h->tmp,      &h->rec,          &h->wbig,   &h->perm,
-                          &h->cell_id,  &h->cell_offsets, &h->pair_count, &h->inv_perm, &h->slots, &h->slotsR, &h->pair_off,   &h->tiles, &h->lookback,
+                          &h->cell_id,  &h->cell_offsets, &h->pair_count, &h->inv_perm, &h->slots, &h->slotsR, &h->pair_off,   &h->tiles, &h->lookback, &h->sysmap,
                           &h->sys_pair_base, &h->first, &h->i,          &h->j,            &h->S,      &h->R,
                           &h->status,   &h->Xsorted,   &h->nb_j,       &h->nb_R,         &h->nb_S,   &h->nb_count};
         for (DevBuf *b : bufs) b->release();

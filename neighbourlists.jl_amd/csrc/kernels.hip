@@ -17,17 +17,31 @@
 namespace nlh {
 
 // ---------------------------------------------------------------------------
+// 0. batches: the system of every atom and every cell, once per build, for
+//    the kernels that would otherwise each search the system tables
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void sys_map_kernel(const int64_t *__restrict__ sys_atom_base,
+                                                      const int32_t *__restrict__ sys_cell_base, int nsys,
+                                                      int64_t nat, int64_t M, int32_t *__restrict__ atom_sys)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < nat) atom_sys[g] = find_sys(sys_atom_base, nsys, g);
+    else if (g < nat + M) atom_sys[g] = find_sys(sys_cell_base, nsys, (int32_t)(g - nat));
+}
+
+// ---------------------------------------------------------------------------
 // 1. binning + cell histogram
 // ---------------------------------------------------------------------------
 
 __global__ __launch_bounds__(256) void bin_kernel(const double *__restrict__ X, const SysDev *__restrict__ sys,
-                                                  const int64_t *__restrict__ sys_atom_base, int nsys, int64_t nat,
+                                                  const int32_t *__restrict__ atom_sys, int nsys, int64_t nat,
                                                   int32_t *__restrict__ key, int32_t *__restrict__ slot,
                                                   int32_t *__restrict__ cell_count, Status *__restrict__ status)
 {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nat) return;
-    const int s = nsys == 1 ? 0 : find_sys(sys_atom_base, nsys, g);
+    const int s = nsys == 1 ? 0 : atom_sys[g];
     const SysDev &sd = sys[s];
     const double x = X[3 * g], y = X[3 * g + 1], z = X[3 * g + 2];
     int32_t c[3], w[3];
@@ -217,7 +231,7 @@ constexpr int kTileSlots = kScanTile + kScanTile / 8;
 template <typename TI>
 __global__ __launch_bounds__(kScanThreads) void scan_cells_kernel(
     int32_t *__restrict__ cell_count, int64_t M, uint64_t *__restrict__ desc, const int64_t *__restrict__ partials,
-    int32_t *__restrict__ cell_start, const SysDev *__restrict__ sys, const int32_t *__restrict__ sys_cell_base,
+    int32_t *__restrict__ cell_start, const SysDev *__restrict__ sys, const int32_t *__restrict__ cell_sys,
     int nsys, void *__restrict__ offsets_out, Status *__restrict__ status)
 {
     const int64_t tile = scan_tile_begin(desc, partials, status);
@@ -249,7 +263,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_cells_kernel(
             const int32_t n0 = cnt_sh[tile_slot(i)];
             cell_start[c] = (int32_t)r0;
             cell_count[c] = 0;
-            const int si = nsys == 1 ? 0 : find_sys(sys_cell_base, nsys, (int32_t)c);
+            const int si = nsys == 1 ? 0 : cell_sys[c];
             const SysDev &sd = sys[si];
             store_ti<TI>(offsets_out, c + si, r0 - sd.atom_base + 1);
             if (c == (int64_t)sd.cell_base + sd.M - 1)   // trailing entry of this system
@@ -367,7 +381,7 @@ __global__ __launch_bounds__(256) void scatter_kernel(const int32_t *__restrict_
 // original index -- the order of Julia's stable sortperm.
 template <typename TI>
 __global__ __launch_bounds__(256) void place_kernel(const double *__restrict__ X, const SysDev *__restrict__ sys,
-                                                    const int32_t *__restrict__ sys_cell_base, int nsys,
+                                                    const int32_t *__restrict__ cell_sys, int nsys,
                                                     const int32_t *__restrict__ key, const int32_t *__restrict__ cell_start,
                                                     const int32_t *__restrict__ tmp, int64_t nat, Rec *__restrict__ rec,
                                                     int32_t *__restrict__ wbig, int32_t *__restrict__ inv_perm,
@@ -381,7 +395,7 @@ __global__ __launch_bounds__(256) void place_kernel(const double *__restrict__ X
     int32_t rank = 0;
     for (int32_t t = cs; t < ce; ++t) rank += tmp[t] < e;
     const int32_t p = cs + rank;
-    const int s = nsys == 1 ? 0 : find_sys(sys_cell_base, nsys, c);
+    const int s = nsys == 1 ? 0 : cell_sys[c];
     const SysDev &sd = sys[s];
     const double x = X[3 * (int64_t)e], y = X[3 * (int64_t)e + 1], z = X[3 * (int64_t)e + 2];
     int32_t cc[3], w[3];
@@ -455,8 +469,11 @@ void launch_clear_pair_lookback(const PipelineBufs &b, hipStream_t st)
 
 void launch_build(const PipelineBufs &b, hipStream_t st)
 {
+    if (b.nsys > 1)
+        hipLaunchKernelGGL(sys_map_kernel, dim3(blocks_for(b.nat + b.M, 256)), dim3(256), 0, st, b.sys_atom_base,
+                           b.sys_cell_base, b.nsys, b.nat, (int64_t)b.M, b.atom_sys);
     if (b.nat > 0)
-        hipLaunchKernelGGL(bin_kernel, dim3(blocks_for(b.nat, 256)), dim3(256), 0, st, b.X, b.sys, b.sys_atom_base,
+        hipLaunchKernelGGL(bin_kernel, dim3(blocks_for(b.nat, 256)), dim3(256), 0, st, b.X, b.sys, b.atom_sys,
                            b.nsys, b.nat, b.key, b.slot, b.cell_count, b.status);
     const int64_t nbc = scan_tiles(b.M);
     int64_t *cpart = nullptr;   // three-kernel scan: tile sums in the (then unused) descriptor slots
@@ -470,13 +487,15 @@ void launch_build(const PipelineBufs &b, hipStream_t st)
     NLH_TI_DISPATCH(b.ti_bytes,
                     hipLaunchKernelGGL(scan_cells_kernel<TI>, dim3((unsigned)nbc), dim3(kScanThreads), 0, st,
                                        b.cell_count, (int64_t)b.M, b.lookback, cpart, b.cell_start, b.sys,
-                                       b.sys_cell_base, b.nsys, b.cell_offsets_out, b.status));
+                                       b.nsys > 1 ? b.atom_sys + b.nat : nullptr, b.nsys, b.cell_offsets_out,
+                                       b.status));
     if (b.nat > 0) {
         hipLaunchKernelGGL(scatter_kernel, dim3(blocks_for(b.nat, 256)), dim3(256), 0, st, b.key, b.slot,
                            b.cell_start, b.nat, b.tmp);
         NLH_TI_DISPATCH(b.ti_bytes,
                         hipLaunchKernelGGL(place_kernel<TI>, dim3(blocks_for(b.nat, 256)), dim3(256), 0, st, b.X,
-                                           b.sys, b.sys_cell_base, b.nsys, b.key, b.cell_start, b.tmp, b.nat, b.rec,
+                                           b.sys, b.nsys > 1 ? b.atom_sys + b.nat : nullptr, b.nsys, b.key,
+                                           b.cell_start, b.tmp, b.nat, b.rec,
                                            b.wbig, b.inv_perm, b.perm_out, b.cell_id_out));
     }
 }

@@ -178,6 +178,7 @@ struct PipelineBufs {
     const int32_t *sys_cell_base;   // nsys + 1
     const int32_t *sys_brick_base;  // nsys + 1
     int32_t nsys;
+    int32_t *atom_sys;          // batches: system of each atom (nat), then of each cell (M); else null
     int64_t nbricks;            // total pair-search bricks
     int64_t nat;                // total atoms
     int32_t M;                  // total cells

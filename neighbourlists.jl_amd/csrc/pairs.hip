@@ -944,7 +944,7 @@ __device__ __forceinline__ void fill_store_one(const RawEntry<WIDE> &e, int32_t 
 
 template <bool WANT_R, bool WIDE, typename TI, int TW>
 __global__ __launch_bounds__(1024) void fill_kernel(
-    const int64_t *__restrict__ sys_atom_base, int nsys, int64_t nat, const int32_t *__restrict__ pair_count,
+    const int64_t *__restrict__ sys_atom_base, const int32_t *__restrict__ atom_sys, int nsys, int64_t nat, const int32_t *__restrict__ pair_count,
     const int64_t *__restrict__ pair_off, const int32_t *__restrict__ tile_atom, int64_t tile_cap,
     const void *__restrict__ slots, const double *__restrict__ slotsR, const Status *__restrict__ status,
     Status *__restrict__ status_pub, int32_t cap, int64_t pair_capacity, TI *__restrict__ i_out,
@@ -996,7 +996,7 @@ __global__ __launch_bounds__(1024) void fill_kernel(
         const int32_t fo0 = -skip0;   // atom A0 owns pair Q0
         const bool batch = nsys > 1;
         int32_t il = 0;   // batches: 1-based system-local index of the lane's atom
-        if (batch && vk) il = (int32_t)(ak - sys_atom_base[find_sys(sys_atom_base, nsys, ak)] + 1);
+        if (batch && vk) il = (int32_t)(ak - sys_atom_base[atom_sys[ak]] + 1);
         // pair q = 64 u + lane belongs to the largest window lane s[u] whose
         // run starts at or before q: one binary search per block, all TW
         // searches advanced together (TW independent lane reads per step),
@@ -1060,7 +1060,7 @@ __global__ __launch_bounds__(1024) void fill_kernel(
             const int r = (int)(P - pair_off[lo]);
             const int64_t rs = lo * (int64_t)cap;
             int32_t isv = (int32_t)(lo + 1);
-            if (nsys > 1) isv = (int32_t)(lo - sys_atom_base[find_sys(sys_atom_base, nsys, lo)] + 1);
+            if (nsys > 1) isv = (int32_t)(lo - sys_atom_base[atom_sys[lo]] + 1);
             const RawEntry<WIDE> e = load_entry<WIDE>(slots, rs, r, (pc & kRow8Bit) != 0);
             fill_store_one<WANT_R, WIDE, TI>(e, isv, rs + r, P, true, slotsR, i_out, j_out, S_out, R_out);
         }
@@ -1105,7 +1105,8 @@ static bool fill_launch(const PipelineBufs &b, hipStream_t st)
     const int64_t waves = (b.pair_capacity + TW * kWave - 1) / (TW * kWave);
     if (waves <= 0) return false;
     const unsigned nblk = (unsigned)((waves * kWave + bs - 1) / bs);
-    hipLaunchKernelGGL((fill_kernel<R, W, TI, TW>), dim3(nblk), dim3(bs), 0, st, b.sys_atom_base, b.nsys, b.nat,
+    hipLaunchKernelGGL((fill_kernel<R, W, TI, TW>), dim3(nblk), dim3(bs), 0, st, b.sys_atom_base, b.atom_sys, b.nsys,
+                       b.nat,
                        b.pair_count, b.pair_off, b.tile_atom, b.tile_cap, b.slots, b.slotsR, b.status, b.status_pub,
                        b.slot_cap, b.pair_capacity, (TI *)b.i_out, (TI *)b.j_out, (TI *)b.S_out, b.R_out);
     return true;
