@@ -179,34 +179,34 @@ def _batched(nsys: int, steps: int):
 def _served(nreq: int, seed: int = 11) -> dict:
     """Request stream (Poisson arrivals; 16..200,000 atoms; 30% rebuilds of a
     resident system; 64 GB budget) through the continuous-batching server and
-    through the plain per-request call.  Open loop at 75% of the plain path's
-    measured capacity for the latencies, then saturation (all requests
-    available at once) for the capacities."""
+    through the plain per-request call, both warmed by one full pass over the
+    stream.  Saturation (all requests available at once) for the capacities,
+    then open loop at 75% of the plain path's capacity for the latencies."""
     from nlhip import serve, workloads
+
+    from nlhip import _lib
 
     reqs = workloads.served_stream(nreq, rate=1.0, seed=seed)
     for r in reqs:
         r.arrival = 0.0
-    warm = reqs[: min(40, nreq)]
-    serve.serve_plain(warm, realtime=False)
-    plain_sat = serve.serve_plain(reqs, realtime=False)
+    # both paths at steady state: one full pass over the stream first (device
+    # buffers grown, resident systems held), as a long-running service is
+    ph = _lib.Handle()
+    serve.serve_plain(reqs, realtime=False, handle=ph)
+    plain_sat = serve.serve_plain(reqs, realtime=False, handle=ph)
     sp = serve.summarize(plain_sat)
+    srv = serve.Server()
+    srv.serve(reqs, realtime=False)
+    served_sat = srv.serve(reqs, realtime=False)
     rate = 0.75 * sp["systems_per_s"]
     gaps = -np.log(1.0 - workloads.uniform(seed + 5, nreq)) / rate
     t = np.cumsum(gaps) - gaps[0]
     for r, a in zip(reqs, t):
         r.arrival = float(a)
-    srv = serve.Server()
-    srv.serve(warm, realtime=False)
     served_open = srv.serve(reqs, realtime=True)
+    plain_open = serve.serve_plain(reqs, realtime=True, handle=ph)
     srv.close()
-    plain_open = serve.serve_plain(reqs, realtime=True)
-    for r in reqs:
-        r.arrival = 0.0
-    srv = serve.Server()
-    srv.serve(warm, realtime=False)     # warm, as the plain path's handle is
-    served_sat = srv.serve(reqs, realtime=False)
-    srv.close()
+    ph.close()
     so, po, ss = serve.summarize(served_open), serve.summarize(plain_open), serve.summarize(served_sat)
     return {"requests": nreq, "mean_atoms": round(float(np.mean([r.nat for r in reqs])), 1),
             "rebuild_frac": round(float(np.mean([r.rebuild for r in reqs])), 3),

@@ -165,8 +165,8 @@ class Server:
     keep_results  copy each result back to the host (PairList) on completion
     """
 
-    def __init__(self, hbm_budget: int = 64 << 30, workers: int = 3, small_atoms: int = 65536,
-                 batch_atoms: int = 1 << 21, int_type=Int32, keep_results: bool = False, max_residents: int = 32):
+    def __init__(self, hbm_budget: int = 64 << 30, workers: int = 4, small_atoms: int = 65536,
+                 batch_atoms: int = 1 << 19, int_type=Int32, keep_results: bool = False, max_residents: int = 32):
         self.budget = int(hbm_budget)
         self.nworkers = int(workers)
         self.small_atoms = int(small_atoms)
@@ -424,11 +424,12 @@ class Server:
 
 
 def serve_plain(requests: list[Request], *, realtime: bool = True, int_type=Int32,
-                keep_results: bool = False) -> list[Completion]:
+                keep_results: bool = False, handle: Handle | None = None) -> list[Completion]:
     """The plain per-request path: one single-system build call per request,
     in arrival order, each waiting for its result (nlh_neighbour_list on
-    host positions)."""
-    h = Handle()
+    host positions), on `handle` (kept by the caller across streams) or a
+    handle of its own."""
+    h = handle or Handle()
     ti = _lib.ti_bytes_of(int_type)
     order = sorted(requests, key=lambda r: r.arrival)
     out = []
@@ -448,7 +449,8 @@ def serve_plain(requests: list[Request], *, realtime: bool = True, int_type=Int3
             res = _collect_pairs(h, g, X, int(n.value), False, None)
         end = time.perf_counter() - t0
         out.append(Completion(r.rid, r.system_id, int(n.value), r.arrival, start, end, 1, res))
-    h.close()
+    if handle is None:
+        h.close()
     return sorted(out, key=lambda c: c.rid)
 
 
