@@ -11,6 +11,9 @@ Also reported on the same JSON line:
                 algorithmic bytes 20 P + 32 nat + 4 (M+1) per launch over its
                 mean duration from HIP events on the build stream
   count_fp64    the distance-test kernel's FP64 rate (11 flop per pair test)
+  count_valu    the same kernel's VALU issue rate: wave64 VALU instructions per
+                launch (rocprofv3 SQ_INSTS_VALU, scaled to the whole chip,
+                profiles/count_valu.json) over its duration measured here
   builds        1M-atom cube and the 1,000,188-atom sheared FCC slab
   batched       4,096 small systems in one batched build vs one call per system
   cpu_baseline  the CPU oracle (C restatement of the reference's sort-based
@@ -35,6 +38,8 @@ import numpy as np  # noqa: E402
 A4500_PAIRS_PER_S = 2.618e6 / 6.9e-3   # README.md:129: 100k atoms, 2.6M pairs, 6.9 ms on an RTX A4500
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
+# one wave64 VALU instruction per SIMD every 4 cycles (16 lanes): 256 CUs x 4 SIMDs x 2.4 GHz / 4
+PEAK_VALU_GIPS = 256 * 4 * 2.4 / 4
 
 
 def _sync():
@@ -128,6 +133,22 @@ def _count_rate(r, cand_per_atom: float) -> dict:
     tflops = tests * 11 / t / 1e12
     return {"kernel": "count_kernel", "bound": "fp64-vector", "achieved": round(tflops, 3), "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s", "frac": round(tflops / PEAK_FP64_TFLOPS, 4), "count_us": round(t * 1e6, 2)}
+
+
+def _count_valu(r) -> dict | None:
+    """VALU issue rate of the count kernel at 1M atoms: the instruction count
+    of one launch (a property of the workload) from the committed counter
+    profile, over the launch's duration measured in this run."""
+    pf = os.path.join(REPO, "profiles", "count_valu.json")
+    if not os.path.exists(pf):
+        return None
+    with open(pf) as f:
+        n = json.load(f)["valu_instructions_per_launch"]
+    t = r["phases_ms"]["count"] * 1e-3
+    gips = n / t / 1e9
+    return {"kernel": "count_kernel", "workload": f"cube {r['nat']} atoms", "bound": "valu-issue",
+            "achieved": round(gips, 1), "peak": PEAK_VALU_GIPS, "unit": "G wave64-instr/s",
+            "frac": round(gips / PEAK_VALU_GIPS, 4), "instructions": n, "count_us": round(t * 1e6, 2)}
 
 
 def _batched(nsys: int, steps: int):
@@ -322,6 +343,7 @@ def main():
         big = _single(c1m, 10, 3, profile=True)
         out["roofline"] = _roofline(big)
         out["count_fp64"] = _count_rate(big, 27 * big["nat"] / big["ncells"])
+        out["count_valu"] = _count_valu(big)
         slab = workloads.triclinic_fcc_slab()
         sl = _single(slab, 10, 3, profile=True)
         out["builds"] = {
