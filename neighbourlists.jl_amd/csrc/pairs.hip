@@ -376,9 +376,11 @@ __device__ __forceinline__ int brick_cell_id(const Brick &B, int hc)
 template <int BT>
 struct BrickLDS {
     using Sh = BrickShape<BT>;
-    double x[Sh::cap], y[Sh::cap], z[Sh::cap];
-    int32_t j[Sh::cap];
-    uint32_t t[Sh::cap];            // pack(s_halo - w_j); home cells have s_halo = 0
+    // entry cap of x, y, z, j, t is a far-away atom that lanes past the last
+    // candidate read, so candidate loads need no per-lane branch
+    double x[Sh::cap + 1], y[Sh::cap + 1], z[Sh::cap + 1];
+    int32_t j[Sh::cap + 1];         // j, or j | S code << 26 when every j fits in 26 bits
+    uint32_t t[Sh::cap + 1];        // pack(s_halo - w_j); home cells have s_halo = 0
     int32_t hoff[Sh::halo + 1];     // LDS offset of each halo cell's atoms
     int32_t hstart[Sh::halo];       // first sorted index of each halo cell
     uint32_t hshift[Sh::halo];      // packed cell shift of each halo cell | kHomeCell
@@ -438,6 +440,9 @@ __device__ bool brick_load(const PairsArgs &A, const Brick &B, BrickLDS<BT> &L)
         L.total = (int32_t)tot;
         L.flag = 0;
         L.row8 = sd.nat > kNarrowMaxAtoms ? 1 : 0;
+        L.x[Sh::cap] = L.y[Sh::cap] = L.z[Sh::cap] = kFar;
+        L.j[Sh::cap] = 0;
+        L.t[Sh::cap] = kPackZero;
     }
     __syncthreads();
     const int total = L.total;
@@ -481,8 +486,11 @@ __device__ bool brick_load(const PairsArgs &A, const Brick &B, BrickLDS<BT> &L)
                                                 : ((unsigned)(tx + 1) <= 2u && (unsigned)(ty + 1) <= 2u &&
                                                    (unsigned)(tz + 1) <= 2u);
             if (!ok) L.row8 = 1;
+            // the 10-bit biased fields of t carry the S codes of a 4-byte entry in
+            // their low 2 bits (meaningful only when the brick stays 4-byte)
+            const uint32_t code = (t & 3u) | ((t >> 8) & 12u) | ((t >> 16) & 48u);
             L.t[idx] = t;
-            L.j[idx] = r[u].j;
+            L.j[idx] = sd.nat <= kNarrowMaxAtoms ? (int32_t)((uint32_t)r[u].j | code << 26) : r[u].j;
             L.x[idx] = r[u].x;
             L.y[idx] = r[u].y;
             L.z[idx] = r[u].z;
@@ -529,74 +537,78 @@ __device__ __forceinline__ HomeRows brick_home(const Brick &B, const BrickLDS<BT
     return R;
 }
 
-// LDS index of candidate k of a home cell (k < total): k plus the delta of
-// the last row whose exclusive prefix is <= k.  The nine (prefix, delta)
-// pairs are wave-uniform.
+// LDS index of candidate k of a home cell (k < total): k plus the offset of
+// its row, the last row whose exclusive prefix is <= k.  The row is found by
+// counting the (wave-uniform) row prefixes at or below k -- an empty row
+// shares its prefix with the next row, which then wins -- and its offset
+// (LDS start - prefix, held by lane r for row r) is fetched by a lane permute,
+// which must run with lanes 0..8 active (an inactive source lane reads 0).
 struct RowTable {
-    int rex[9];   // exclusive candidate prefix of each row
-    int d0;       // LDS start of row 0
-    int inc[9];   // change of (LDS start - prefix) from row r - 1 to row r
+    int rex[9];   // exclusive candidate prefix of each row (uniform)
+    int dl;       // lane r < 9: LDS start of row r minus its prefix
 };
 
 __device__ __forceinline__ RowTable row_table(const HomeRows &R)
 {
     RowTable T;
-    int prev = 0;
 #pragma unroll
-    for (int r = 0; r < 9; ++r) {
-        T.rex[r] = __builtin_amdgcn_readlane(R.rex, r);
-        const int dl = __builtin_amdgcn_readlane(R.rstart, r) - T.rex[r];
-        T.inc[r] = dl - prev;
-        prev = dl;
-    }
-    T.d0 = T.inc[0];
+    for (int r = 0; r < 9; ++r) T.rex[r] = __builtin_amdgcn_readlane(R.rex, r);
+    T.dl = R.rstart - R.rex;
     return T;
 }
 
 __device__ __forceinline__ int row_index(const RowTable &T, int k)
 {
-    int d = T.d0;
+    int row = 0;
 #pragma unroll
-    for (int r = 1; r < 9; ++r) d += k >= T.rex[r] ? T.inc[r] : 0;
-    return k + d;
+    for (int r = 1; r < 9; ++r) row += k >= T.rex[r] ? 1 : 0;
+    return k + __builtin_amdgcn_ds_bpermute(row << 2, T.dl);
 }
 
 // Append one ballot's hits to a home atom's slot row.  `row` and `rrow` are
 // the (wave-uniform) slot row and its R row; ca the atom's running count;
-// ent = j | pack(s - w_j) << 32 is the candidate's entry when w_i = 0.
+// m the ballot of the hit lanes (the self pair already cleared), which also
+// serves as the store mask; ent = j | pack(s - w_j) << 32 is the candidate's
+// entry when w_i = 0.  Entry r of the row is written only while r < cap; the
+// per-lane bound is tested only when the row can overflow in this ballot.
 template <bool NARROW, bool WZERO, bool WANT_R, bool WIDE>
 __device__ __forceinline__ void emit_row(void *__restrict__ row, double *__restrict__ rrow, uint32_t cap,
-                                         Status *__restrict__ status, bool hit, uint64_t m, int32_t &ca, uint64_t ent,
+                                         Status *__restrict__ status, uint64_t m, int32_t &ca, uint64_t ent,
                                          int32_t wix, int32_t wiy, int32_t wiz, double Rx, double Ry, double Rz)
 {
-    const uint32_t r =
-        __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) + (uint32_t)ca;
-    if (hit && r < cap) {
-        if (NARROW) {
-            // j | S code << 26 (w_i = 0 in a 4-byte brick)
-            reinterpret_cast<uint32_t *>(row)[r] = (uint32_t)ent;
-        } else if (WIDE) {
-            int32_t tx, ty, tz;
-            unpack_wind((uint32_t)(ent >> 32), tx, ty, tz);
-            reinterpret_cast<int4 *>(row)[r] = make_int4((int32_t)(uint32_t)ent, tx + wix, ty + wiy, tz + wiz);
-        } else if (WZERO) {
-            // S = s_loop + w_i - w_j = s_loop - w_j  (gpu_kernels.jl:88)
-            reinterpret_cast<uint64_t *>(row)[r] = ent;
-        } else {
-            int32_t tx, ty, tz;
-            unpack_wind((uint32_t)(ent >> 32), tx, ty, tz);
-            const uint32_t spk = pack_wind(tx + wix, ty + wiy, tz + wiz);
-            if (spk & kWindBig) status->wide = 1;
-            reinterpret_cast<uint64_t *>(row)[r] = (ent & 0xffffffffull) | ((uint64_t)spk << 32);
+    const uint32_t n = (uint32_t)__popcll(m);
+    if (n != 0) {
+        const uint32_t r =
+            __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) + (uint32_t)ca;
+        uint64_t st = m;
+        if ((uint32_t)ca + n > cap) st &= __builtin_amdgcn_ballot_w64(r < cap);
+        if (__builtin_amdgcn_inverse_ballot_w64(st)) {
+            if (NARROW) {
+                // j | S code << 26 (w_i = 0 in a 4-byte brick)
+                *reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(row) + (r << 2)) = (uint32_t)ent;
+            } else if (WIDE) {
+                int32_t tx, ty, tz;
+                unpack_wind((uint32_t)(ent >> 32), tx, ty, tz);
+                reinterpret_cast<int4 *>(row)[r] = make_int4((int32_t)(uint32_t)ent, tx + wix, ty + wiy, tz + wiz);
+            } else if (WZERO) {
+                // S = s_loop + w_i - w_j = s_loop - w_j  (gpu_kernels.jl:88)
+                reinterpret_cast<uint64_t *>(row)[r] = ent;
+            } else {
+                int32_t tx, ty, tz;
+                unpack_wind((uint32_t)(ent >> 32), tx, ty, tz);
+                const uint32_t spk = pack_wind(tx + wix, ty + wiy, tz + wiz);
+                if (spk & kWindBig) status->wide = 1;
+                reinterpret_cast<uint64_t *>(row)[r] = (ent & 0xffffffffull) | ((uint64_t)spk << 32);
+            }
+            if (WANT_R) {
+                double *Rd = rrow + 3 * r;
+                Rd[0] = Rx;
+                Rd[1] = Ry;
+                Rd[2] = Rz;
+            }
         }
-        if (WANT_R) {
-            double *Rd = rrow + 3 * r;
-            Rd[0] = Rx;
-            Rd[1] = Ry;
-            Rd[2] = Rz;
-        }
+        ca += (int32_t)n;
     }
-    ca += __popcll(m);
 }
 
 // One (home atom, chunk) test of the brick pass.  SHIFT: add the candidate's
@@ -622,8 +634,7 @@ __device__ __forceinline__ void test_chunk(double cx, double cy, double cz, doub
     // slot in the home cell, lane dself of this chunk (if it is in range)
     const uint64_t self = (unsigned)dself < (unsigned)kWave ? ~(1ull << dself) : ~0ull;
     const uint64_t m = __builtin_amdgcn_ballot_w64(in) & self;
-    emit_row<NARROW, WZERO, WANT_R, WIDE>(row, rrow, cap, status, in && lane_id() != dself, m, ca, ent, wix, wiy, wiz,
-                                          Rx, Ry, Rz);
+    emit_row<NARROW, WZERO, WANT_R, WIDE>(row, rrow, cap, status, m, ca, ent, wix, wiy, wiz, Rx, Ry, Rz);
 }
 
 // All chunks of one home atom.
@@ -683,10 +694,8 @@ __device__ __forceinline__ void test_home_pair(int nch, const double *cx, const 
             const uint64_t self1 = (unsigned)d1 < (unsigned)kWave ? ~(1ull << d1) : ~0ull;
             const uint64_t m0 = __builtin_amdgcn_ballot_w64(in0) & self0;
             const uint64_t m1 = __builtin_amdgcn_ballot_w64(in1) & self1;
-            emit_row<true, true, WANT_R, WIDE>(row0, rrow0, cap, status, in0 && lane_id() != d0, m0, ca0, ent[g], 0, 0,
-                                               0, R0x, R0y, R0z);
-            emit_row<true, true, WANT_R, WIDE>(row1, rrow1, cap, status, in1 && lane_id() != d1, m1, ca1, ent[g], 0, 0,
-                                               0, R1x, R1y, R1z);
+            emit_row<true, true, WANT_R, WIDE>(row0, rrow0, cap, status, m0, ca0, ent[g], 0, 0, 0, R0x, R0y, R0z);
+            emit_row<true, true, WANT_R, WIDE>(row1, rrow1, cap, status, m1, ca1, ent[g], 0, 0, 0, R1x, R1y, R1z);
         }
     }
 }
@@ -724,16 +733,19 @@ __global__ __launch_bounds__(BT) void count_kernel(
     const int lane = lane_id();
     const int wv = sgpr((int)(threadIdx.x >> 6));
     const bool narrow = !WIDE && sgpr(L.row8) == 0;   // the brick writes 4-byte slot entries
+    // L.j holds j | S code << 26 whenever every j fits in 26 bits
+    const uint32_t hjmask = sd.nat <= kNarrowMaxAtoms ? (uint32_t)(kNarrowMaxAtoms - 1) : 0xffffffffu;
+    const uint32_t jmask = narrow ? 0xffffffffu : hjmask;   // entry low word: j | code << 26, or j
     for (int hc = wv; hc < B.nhome; hc += BrickShape<BT>::waves) {
         const HomeRows R = brick_home(B, L, hc);
         const RowTable T = row_table(R);
         const int nha = R.h1 - R.h0;
         for (int hg = 0; hg < nha; hg += kWave) {
             const int nh = sgpr(min(kWave, nha - hg));
-            const int hl = R.h0 + hg + (lane < nh ? lane : 0);
-            const double hx_l = L.x[hl], hy_l = L.y[hl], hz_l = L.z[hl];
+            const int hb = R.h0 + hg;        // LDS index of home atom hg (uniform)
+            const int hl = hb + (lane < nh ? lane : 0);
             const uint32_t ht_l = L.t[hl];   // pack(-w_i): a home cell's own shift is zero
-            const int32_t hj_l = L.j[hl];
+            const int32_t hj_l = (int32_t)((uint32_t)L.j[hl] & hjmask);
             const int kself = R.self0 + hg;  // candidate index of home atom hg + a is kself + a
             int32_t cnt_l = 0;                // lane a: running count of home atom hg + a
             for (int k0 = 0; k0 < R.total; k0 += kBrickChunks * kWave) {
@@ -743,23 +755,16 @@ __global__ __launch_bounds__(BT) void count_kernel(
                 uint64_t ent[kBrickChunks];   // j | pack(s - w_j) << 32
 #pragma unroll
                 for (int g = 0; g < kBrickChunks; ++g) {
-                    cx[g] = cy[g] = cz[g] = kFar;
-                    ent[g] = (uint64_t)kPackZero << 32;
-                    if (g < nch) {
+                    if (g < nch) {   // chunks past nch are never read
                         const int k = k0 + g * kWave + lane;
-                        const bool v = k < R.total;
-                        const int li = row_index(T, v ? k : 0);
-                        if (v) {
-                            cx[g] = L.x[li];
-                            cy[g] = L.y[li];
-                            cz[g] = L.z[li];
-                            const uint32_t t = L.t[li];
-                            // low word: j, or j | S code << 26 for 4-byte entries; the
-                            // 10-bit biased fields of t are the codes in their low 2 bits
-                            const uint32_t code = (t & 3u) | ((t >> 8) & 12u) | ((t >> 16) & 48u);
-                            const uint32_t lo = (uint32_t)L.j[li] | (narrow ? code << 26 : 0u);
-                            ent[g] = (uint64_t)lo | ((uint64_t)t << 32);
-                        }
+                        // every lane takes part in row_index's permute: a lane reads
+                        // only from lanes that are active
+                        const int lr = row_index(T, k);
+                        const int li = k < R.total ? lr : BrickShape<BT>::cap;   // else the far sentinel
+                        cx[g] = L.x[li];
+                        cy[g] = L.y[li];
+                        cz[g] = L.z[li];
+                        ent[g] = (uint64_t)((uint32_t)L.j[li] & jmask) | ((uint64_t)L.t[li] << 32);
                     }
                 }
                 uint32_t wcur = 0xffffffffu;   // home winding the shifts were computed for
@@ -782,10 +787,9 @@ __global__ __launch_bounds__(BT) void count_kernel(
                     }
                     wcur = kPackZero;
                     for (; a + 1 < nh; a += 2) {
-                        const double h0x = readlane_f64(hx_l, a), h0y = readlane_f64(hy_l, a),
-                                     h0z = readlane_f64(hz_l, a);
-                        const double h1x = readlane_f64(hx_l, a + 1), h1y = readlane_f64(hy_l, a + 1),
-                                     h1z = readlane_f64(hz_l, a + 1);
+                        // uniform LDS reads: every lane receives the home atom's coordinates
+                        const double h0x = L.x[hb + a], h0y = L.y[hb + a], h0z = L.z[hb + a];
+                        const double h1x = L.x[hb + a + 1], h1y = L.y[hb + a + 1], h1z = L.z[hb + a + 1];
                         int32_t ca0 = __builtin_amdgcn_readlane(cnt_l, a), ca1 = __builtin_amdgcn_readlane(cnt_l, a + 1);
                         const int64_t srow0 = (sd.atom_base + __builtin_amdgcn_readlane(hj_l, a)) * (int64_t)cap;
                         const int64_t srow1 = (sd.atom_base + __builtin_amdgcn_readlane(hj_l, a + 1)) * (int64_t)cap;
@@ -802,7 +806,7 @@ __global__ __launch_bounds__(BT) void count_kernel(
                     }
                 }
                 for (; a < nh; ++a) {
-                    const double hx = readlane_f64(hx_l, a), hy = readlane_f64(hy_l, a), hz = readlane_f64(hz_l, a);
+                    const double hx = L.x[hb + a], hy = L.y[hb + a], hz = L.z[hb + a];
                     const uint32_t ht = (uint32_t)__builtin_amdgcn_readlane((int)ht_l, a);
                     if (ht != wcur) {   // first home atom, or a new home winding
                         int32_t tx, ty, tz;
