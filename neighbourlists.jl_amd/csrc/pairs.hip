@@ -571,17 +571,38 @@ __device__ __forceinline__ int row_index(const RowTable &T, int k)
 // serves as the store mask; ent = j | pack(s - w_j) << 32 is the candidate's
 // entry when w_i = 0.  Entry r of the row is written only while r < cap; the
 // per-lane bound is tested only when the row can overflow in this ballot.
+//
+// 4-byte rows without R go through a buffer resource spanning the row's cap
+// entries: every lane stores, lanes without a hit at an offset past the
+// resource's end, and the hardware's range check drops those stores and any
+// entry at r >= cap.  No exec-mask or bound branches.
+constexpr uint32_t kDropOffset = 0x80000000u;   // beyond any row resource (cap * 4 < 2^31)
+constexpr int32_t kRawBufferWord3 = 0x00020000; // raw 32-bit buffer access (GFX9 family)
+
 template <bool NARROW, bool WZERO, bool WANT_R, bool WIDE>
 __device__ __forceinline__ void emit_row(void *__restrict__ row, double *__restrict__ rrow, uint32_t cap,
                                          Status *__restrict__ status, uint64_t m, int32_t &ca, uint64_t ent,
                                          int32_t wix, int32_t wiy, int32_t wiz, double Rx, double Ry, double Rz)
 {
     const uint32_t n = (uint32_t)__popcll(m);
+    if (NARROW && !WANT_R) {
+        if (n != 0) {
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(row, 0, (int)(cap * 4u), kRawBufferWord3);
+            const uint32_t rk =
+                __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            const uint32_t off = __builtin_amdgcn_inverse_ballot_w64(m) ? (rk + (uint32_t)ca) << 2 : kDropOffset;
+            // j | S code << 26 (w_i = 0 in a 4-byte brick)
+            __builtin_amdgcn_raw_buffer_store_b32((uint32_t)ent, rs, (int)off, 0, 0);
+            ca += (int32_t)n;
+        }
+        return;
+    }
     if (n != 0) {
-        const uint32_t r =
-            __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) + (uint32_t)ca;
+        const uint32_t rk = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
         uint64_t st = m;
-        if ((uint32_t)ca + n > cap) st &= __builtin_amdgcn_ballot_w64(r < cap);
+        if ((uint32_t)ca + n > cap)   // lanes whose entry would fall past the row's end store nothing
+            st &= __builtin_amdgcn_ballot_w64((int32_t)rk < (int32_t)cap - ca);
+        const uint32_t r = rk + (uint32_t)ca;
         if (__builtin_amdgcn_inverse_ballot_w64(st)) {
             if (NARROW) {
                 // j | S code << 26 (w_i = 0 in a 4-byte brick)
@@ -613,11 +634,11 @@ __device__ __forceinline__ void emit_row(void *__restrict__ row, double *__restr
 
 // One (home atom, chunk) test of the brick pass.  SHIFT: add the candidate's
 // cell shift C' S (left out when every lane's shift is zero and R is not
-// written: r^2 is then the same).  `dself` is the lane of the home atom's own
-// (j == i, s == 0) candidate in this chunk, or out of range.
+// written: r^2 is then the same).  `self` clears the home atom's own
+// (j == i, s == 0) candidate from the ballot when it lies in this chunk.
 template <bool NARROW, bool SHIFT, bool WZERO, bool WANT_R, bool WIDE>
 __device__ __forceinline__ void test_chunk(double cx, double cy, double cz, double sx, double sy, double sz, double hx,
-                                           double hy, double hz, double rc2, int dself, void *__restrict__ row,
+                                           double hy, double hz, double rc2, uint64_t self, void *__restrict__ row,
                                            double *__restrict__ rrow, uint32_t cap, Status *__restrict__ status,
                                            int32_t &ca, uint64_t ent, int32_t wix, int32_t wiy, int32_t wiz)
 {
@@ -630,9 +651,6 @@ __device__ __forceinline__ void test_chunk(double cx, double cy, double cz, doub
     }
     const double r2 = (Rx * Rx + Ry * Ry) + Rz * Rz;
     const bool in = r2 < rc2;
-    // _is_self_interaction: j == i with no cell shift is the home atom's own
-    // slot in the home cell, lane dself of this chunk (if it is in range)
-    const uint64_t self = (unsigned)dself < (unsigned)kWave ? ~(1ull << dself) : ~0ull;
     const uint64_t m = __builtin_amdgcn_ballot_w64(in) & self;
     emit_row<NARROW, WZERO, WANT_R, WIDE>(row, rrow, cap, status, m, ca, ent, wix, wiy, wiz, Rx, Ry, Rz);
 }
@@ -640,22 +658,27 @@ __device__ __forceinline__ void test_chunk(double cx, double cy, double cz, doub
 // All chunks of one home atom.
 template <bool NARROW, bool WZERO, bool WANT_R, bool WIDE>
 __device__ __forceinline__ void test_home_atom(int nch, const double *cx, const double *cy, const double *cz,
-                                               const double *sx, const double *sy, const double *sz, const bool *csh,
+                                               const double *sx, const double *sy, const double *sz, const uint64_t *csh,
                                                const uint64_t *ent, double hx, double hy, double hz, double rc2,
                                                int ks, void *__restrict__ row, double *__restrict__ rrow, uint32_t cap,
                                                Status *__restrict__ status, int32_t &ca, int32_t wix, int32_t wiy,
                                                int32_t wiz)
 {
+    // _is_self_interaction: j == i with no cell shift is the home atom's own
+    // slot in the home cell, candidate ks of this pass: chunk ks >> 6 (none
+    // when ks is out of the pass), lane ks & 63
+    const int gs = ks >> 6;
+    const uint64_t nself = ~(1ull << (ks & (kWave - 1)));
 #pragma unroll
     for (int g = 0; g < kBrickChunks; ++g) {
         if (g < nch) {
-            const int dself = ks - g * kWave;
-            if (WANT_R || csh[g])
+            const uint64_t self = g == gs ? nself : ~0ull;
+            if (WANT_R || csh[g] != 0)
                 test_chunk<NARROW, true, WZERO, WANT_R, WIDE>(cx[g], cy[g], cz[g], sx[g], sy[g], sz[g], hx, hy, hz, rc2,
-                                                              dself, row, rrow, cap, status, ca, ent[g], wix, wiy, wiz);
+                                                              self, row, rrow, cap, status, ca, ent[g], wix, wiy, wiz);
             else
                 test_chunk<NARROW, false, WZERO, WANT_R, WIDE>(cx[g], cy[g], cz[g], sx[g], sy[g], sz[g], hx, hy, hz, rc2,
-                                                               dself, row, rrow, cap, status, ca, ent[g], wix, wiy, wiz);
+                                                               self, row, rrow, cap, status, ca, ent[g], wix, wiy, wiz);
         }
     }
 }
@@ -666,20 +689,23 @@ __device__ __forceinline__ void test_home_atom(int nch, const double *cx, const 
 // Atom 1's self candidate follows atom 0's.
 template <bool WANT_R, bool WIDE>
 __device__ __forceinline__ void test_home_pair(int nch, const double *cx, const double *cy, const double *cz,
-                                               const double *sx, const double *sy, const double *sz, const bool *csh,
+                                               const double *sx, const double *sy, const double *sz, const uint64_t *csh,
                                                const uint64_t *ent, double h0x, double h0y, double h0z, double h1x,
                                                double h1y, double h1z, double rc2, int ks0, void *__restrict__ row0,
                                                void *__restrict__ row1, double *__restrict__ rrow0,
                                                double *__restrict__ rrow1, uint32_t cap, Status *__restrict__ status,
                                                int32_t &ca0, int32_t &ca1)
 {
+    // self candidates (as in test_home_atom): ks0 for atom 0, ks0 + 1 for atom 1
+    const int gs0 = ks0 >> 6, gs1 = (ks0 + 1) >> 6;
+    const uint64_t nself0 = ~(1ull << (ks0 & (kWave - 1))), nself1 = ~(1ull << ((ks0 + 1) & (kWave - 1)));
 #pragma unroll
     for (int g = 0; g < kBrickChunks; ++g) {
         if (g < nch) {
             // R = xj - xi + C' * S ; r^2 = dot(R, R)  (gpu_kernels.jl:89-92)
             double R0x = cx[g] - h0x, R0y = cy[g] - h0y, R0z = cz[g] - h0z;
             double R1x = cx[g] - h1x, R1y = cy[g] - h1y, R1z = cz[g] - h1z;
-            if (WANT_R || csh[g]) {
+            if (WANT_R || csh[g] != 0) {
                 R0x = R0x + sx[g];
                 R0y = R0y + sy[g];
                 R0z = R0z + sz[g];
@@ -689,11 +715,8 @@ __device__ __forceinline__ void test_home_pair(int nch, const double *cx, const 
             }
             const bool in0 = (R0x * R0x + R0y * R0y) + R0z * R0z < rc2;
             const bool in1 = (R1x * R1x + R1y * R1y) + R1z * R1z < rc2;
-            const int d0 = ks0 - g * kWave, d1 = d0 + 1;
-            const uint64_t self0 = (unsigned)d0 < (unsigned)kWave ? ~(1ull << d0) : ~0ull;
-            const uint64_t self1 = (unsigned)d1 < (unsigned)kWave ? ~(1ull << d1) : ~0ull;
-            const uint64_t m0 = __builtin_amdgcn_ballot_w64(in0) & self0;
-            const uint64_t m1 = __builtin_amdgcn_ballot_w64(in1) & self1;
+            const uint64_t m0 = __builtin_amdgcn_ballot_w64(in0) & (g == gs0 ? nself0 : ~0ull);
+            const uint64_t m1 = __builtin_amdgcn_ballot_w64(in1) & (g == gs1 ? nself1 : ~0ull);
             emit_row<true, true, WANT_R, WIDE>(row0, rrow0, cap, status, m0, ca0, ent[g], 0, 0, 0, R0x, R0y, R0z);
             emit_row<true, true, WANT_R, WIDE>(row1, rrow1, cap, status, m1, ca1, ent[g], 0, 0, 0, R1x, R1y, R1z);
         }
@@ -769,20 +792,24 @@ __global__ __launch_bounds__(BT) void count_kernel(
                 }
                 uint32_t wcur = 0xffffffffu;   // home winding the shifts were computed for
                 int32_t wix = 0, wiy = 0, wiz = 0;
-                bool csh[kBrickChunks];        // wave-uniform: some lane of the chunk has a nonzero shift
+                // wave-uniform, per chunk: the lanes whose shift is nonzero (the
+                // shifts sx, sy, sz are set only where this is nonzero or R is written)
+                uint64_t csh[kBrickChunks];
                 int a = 0;
                 if (narrow && nh >= 2) {
                     // every home atom of a 4-byte brick has w_i = 0: the shifts of
                     // the candidates are their own cell shifts
 #pragma unroll
                     for (int g = 0; g < kBrickChunks; ++g) {
-                        sx[g] = sy[g] = sz[g] = 0.0;
-                        csh[g] = false;
+                        csh[g] = 0;
                         if (g < nch) {
-                            int32_t cx_, cy_, cz_;
-                            unpack_wind((uint32_t)(ent[g] >> 32), cx_, cy_, cz_);
-                            csh[g] = __builtin_amdgcn_ballot_w64((cx_ | cy_ | cz_) != 0) != 0;
-                            if (csh[g]) cell_shift(Csh, cx_, cy_, cz_, sx[g], sy[g], sz[g]);
+                            const uint32_t t = (uint32_t)(ent[g] >> 32);
+                            csh[g] = __builtin_amdgcn_ballot_w64(t != kPackZero);
+                            if (WANT_R || csh[g] != 0) {
+                                int32_t cx_, cy_, cz_;
+                                unpack_wind(t, cx_, cy_, cz_);
+                                cell_shift(Csh, cx_, cy_, cz_, sx[g], sy[g], sz[g]);
+                            }
                         }
                     }
                     wcur = kPackZero;
@@ -802,7 +829,8 @@ __global__ __launch_bounds__(BT) void count_kernel(
                         test_home_pair<WANT_R, WIDE>(nch, cx, cy, cz, sx, sy, sz, csh, ent, h0x, h0y, h0z, h1x, h1y,
                                                      h1z, rc2, kself + a - k0, row0, row1, rrow0, rrow1,
                                                      (uint32_t)cap, status, ca0, ca1);
-                        cnt_l = lane == a ? ca0 : (lane == a + 1 ? ca1 : cnt_l);
+                        cnt_l = __builtin_amdgcn_inverse_ballot_w64(1ull << a) ? ca0 : cnt_l;
+                        cnt_l = __builtin_amdgcn_inverse_ballot_w64(2ull << a) ? ca1 : cnt_l;
                     }
                 }
                 for (; a < nh; ++a) {
@@ -814,14 +842,13 @@ __global__ __launch_bounds__(BT) void count_kernel(
                         wix = -tx; wiy = -ty; wiz = -tz;
 #pragma unroll
                         for (int g = 0; g < kBrickChunks; ++g) {
-                            sx[g] = sy[g] = sz[g] = 0.0;
-                            csh[g] = false;
+                            csh[g] = 0;
                             if (g < nch) {
                                 int32_t cx_, cy_, cz_;
                                 unpack_wind((uint32_t)(ent[g] >> 32), cx_, cy_, cz_);
                                 cx_ += wix; cy_ += wiy; cz_ += wiz;
-                                csh[g] = __builtin_amdgcn_ballot_w64((cx_ | cy_ | cz_) != 0) != 0;
-                                if (csh[g]) cell_shift(Csh, cx_, cy_, cz_, sx[g], sy[g], sz[g]);
+                                csh[g] = __builtin_amdgcn_ballot_w64((cx_ | cy_ | cz_) != 0);
+                                if (WANT_R || csh[g] != 0) cell_shift(Csh, cx_, cy_, cz_, sx[g], sy[g], sz[g]);
                             }
                         }
                         wcur = ht;

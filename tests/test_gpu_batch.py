@@ -28,16 +28,17 @@ def _jitter(X, seed, sigma):
     return X + sigma * workloads.normal(seed, X.size).reshape(X.shape)
 
 
-def _check_oracle(res, systems, Xcat, offs, which):
+def _check_oracle(res, systems, Xcat, offs, which, R=True):
     for s in which:
         sysm = systems[s]
         Xs = Xcat[offs[s]:offs[s + 1]]
-        ref = oracle.neighbour_list(Xs, sysm.cutoff, sysm.cell, sysm.pbc, want_R=True)
+        ref = oracle.neighbour_list(Xs, sysm.cutoff, sysm.cell, sysm.pbc, want_R=R)
         pl = res.pairlist(s)
         assert pl.npairs() == ref.npairs
         assert np.array_equal(pl.i, ref.i) and np.array_equal(pl.j, ref.j) and np.array_equal(pl.S, ref.S)
         assert np.array_equal(pl.first, ref.first)
-        assert np.array_equal(np.asarray(pl.R).view(np.int64), ref.R.view(np.int64))
+        if R:
+            assert np.array_equal(np.asarray(pl.R).view(np.int64), ref.R.view(np.int64))
 
 
 def test_graph_replays_match_eager():
@@ -91,6 +92,10 @@ def test_graph_replay_outgrows_capacity():
     _check_oracle(g, dense, Xd_, offs, range(24))
     g2 = bb.rebuild(Xd_, want_R=True)                       # re-captured on the grown buffers
     _same(g, g2)
+    plain = BatchBuilder()                                  # the same without R
+    plain.run(systems)
+    plain.rebuild(X0)
+    _check_oracle(plain.rebuild(Xd_), dense, Xd_, offs, range(24), R=False)
 
 
 @pytest.mark.parametrize("brick_atoms,fill_block,fill_pairs,count_block",
@@ -135,3 +140,5 @@ def test_autotune_keeps_results():
     after = bb.rebuild(X0, want_R=True)
     _same(before, after)
     _check_oracle(after, systems, X0, offs, [0, 1, 64, 127])
+    plain = bb.rebuild(X0)                      # the tuned batch without R
+    _check_oracle(plain, systems, X0, offs, [0, 1, 64, 127], R=False)

@@ -33,11 +33,14 @@ def _same(nl, ref, R=True):
 
 
 def _check(X, rc, C, pbc, int_type=np.int32):
+    """Both builds: with R, and without (4-byte slot rows written through
+    range-checked buffer stores, the path the benchmarks time)."""
     ti = 32 if np.dtype(int_type) == np.int32 else 64
     nl = nlhip.neighbour_list(X, rc, C, pbc, int_type=int_type, want_R=True)
     ref = oracle.neighbour_list(X, rc, C, pbc, ti_bits=ti, want_R=True)
     assert nl.i.dtype == np.dtype(int_type)
     _same(nl, ref)
+    _same(nlhip.neighbour_list(X, rc, C, pbc, int_type=int_type), ref, R=False)
     return nl, ref
 
 
@@ -173,6 +176,21 @@ def test_bricks_medium(pbc):
     s = workloads.random_cubic(20000, density=0.05, seed=61)
     print(f"bricks pbc={pbc}", flush=True)
     _check(s.X, 5.0, s.cell, pbc)
+
+
+def test_slot_rows_overflow_and_regrow():
+    # 80% of the atoms packed into one eighth of the box: the slot rows sized
+    # from the mean density (about 8 neighbours, 32 slots) are too short for
+    # the clustered atoms (about 50), so the count drops the entries past each
+    # row's end and the build repeats on longer rows
+    rng = np.random.default_rng(71)
+    L, n = 60.0, 20000
+    nc = int(0.8 * n)
+    X = np.concatenate([rng.random((nc, 3)) * (L / 2), rng.random((n - nc, 3)) * L])
+    C = np.eye(3) * L
+    nl, ref = _check(X, 2.74, C, (True, True, True))
+    assert nlhip.max_neighbours(nl) > 40
+    _check(X, 2.74, C, (False, True, False))
 
 
 def test_bricks_dense_triclinic():
