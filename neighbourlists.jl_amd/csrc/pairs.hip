@@ -528,10 +528,16 @@ __device__ __forceinline__ HomeRows brick_home(const Brick &B, const BrickLDS<BT
         st = L.hoff[hr];
         len = L.hoff[hr + 3] - st;
     }
-    const int incl = wave_incl_scan(len);
+    // inclusive scan over lanes 0..8 (four steps reach back 15 lanes)
+    int incl = len;
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+        const int t = __shfl_up(incl, off, kWave);
+        if (lane >= off) incl += t;
+    }
     R.rex = incl - len;
     R.rstart = st;
-    R.total = sgpr(__shfl(incl, kWave - 1, kWave));
+    R.total = __builtin_amdgcn_readlane(incl, 8);
     // row 4 is (dz, dy) = (0, 0); the home cell is its middle cell
     R.self0 = __builtin_amdgcn_readlane(R.rex, 4) + (R.h0 - __builtin_amdgcn_readlane(R.rstart, 4));
     return R;
@@ -563,6 +569,13 @@ __device__ __forceinline__ int row_index(const RowTable &T, int k)
 #pragma unroll
     for (int r = 1; r < 9; ++r) row += k >= T.rex[r] ? 1 : 0;
     return k + __builtin_amdgcn_ds_bpermute(row << 2, T.dl);
+}
+
+// First slot of atom e's row: both factors are below 2^31, so one 32 x 32 -> 64
+// bit product.
+__device__ __forceinline__ int64_t slot_row(int64_t e, int32_t cap)
+{
+    return (int64_t)((uint64_t)(uint32_t)e * (uint32_t)cap);
 }
 
 // Append one ballot's hits to a home atom's slot row.  `row` and `rrow` are
@@ -818,8 +831,8 @@ __global__ __launch_bounds__(BT) void count_kernel(
                         const double h0x = L.x[hb + a], h0y = L.y[hb + a], h0z = L.z[hb + a];
                         const double h1x = L.x[hb + a + 1], h1y = L.y[hb + a + 1], h1z = L.z[hb + a + 1];
                         int32_t ca0 = __builtin_amdgcn_readlane(cnt_l, a), ca1 = __builtin_amdgcn_readlane(cnt_l, a + 1);
-                        const int64_t srow0 = (sd.atom_base + __builtin_amdgcn_readlane(hj_l, a)) * (int64_t)cap;
-                        const int64_t srow1 = (sd.atom_base + __builtin_amdgcn_readlane(hj_l, a + 1)) * (int64_t)cap;
+                        const int64_t srow0 = slot_row(sd.atom_base + __builtin_amdgcn_readlane(hj_l, a), cap);
+                        const int64_t srow1 = slot_row(sd.atom_base + __builtin_amdgcn_readlane(hj_l, a + 1), cap);
                         void *row0 = WIDE ? (void *)(reinterpret_cast<int4 *>(slots) + srow0)
                                           : (void *)(reinterpret_cast<uint64_t *>(slots) + srow0);
                         void *row1 = WIDE ? (void *)(reinterpret_cast<int4 *>(slots) + srow1)
@@ -856,7 +869,7 @@ __global__ __launch_bounds__(BT) void count_kernel(
                     const bool wzero = ht == kPackZero;   // the home atom lies inside the cell
                     int32_t ca = __builtin_amdgcn_readlane(cnt_l, a);
                     // slot rows are indexed by original atom index, so the fill reads them in order
-                    const int64_t srow = (sd.atom_base + __builtin_amdgcn_readlane(hj_l, a)) * (int64_t)cap;
+                    const int64_t srow = slot_row(sd.atom_base + __builtin_amdgcn_readlane(hj_l, a), cap);
                     void *row = WIDE ? (void *)(reinterpret_cast<int4 *>(slots) + srow)
                                      : (void *)(reinterpret_cast<uint64_t *>(slots) + srow);
                     double *rrow = WANT_R ? slotsR + 3 * srow : nullptr;
