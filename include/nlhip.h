@@ -171,7 +171,10 @@ int nlh_batch_rebuild(nlh_handle *h, const double *X_batch, uint32_t flags, int6
  * fill wave (128, 256 or 512; each wave writes one aligned tile of the
  * pair arrays; default 256).  count_block: threads per count-pass brick workgroup, 256 or
  * 512, or 0 for the default (256); a 256-thread brick holds at most 768
- * expected atoms (default brick_atoms 768).
+ * expected atoms (default brick_atoms 768).  cell_sort: 0 (default) sorts
+ * the atoms by cell through buckets of cells counted in LDS wherever the
+ * cell grid allows (up to 8192 buckets of 4096 cells), 1 uses one global
+ * atomic histogram over the cells.
  * Every configuration gives identical results; no reference counterpart
  * (KernelAbstractions picks workgroup sizes itself). */
 typedef struct {
@@ -179,6 +182,7 @@ typedef struct {
     int32_t fill_block;
     int32_t fill_pairs;
     int32_t count_block;
+    int32_t cell_sort;
 } nlh_tuning;
 int nlh_set_tuning(nlh_handle *h, const nlh_tuning *t);
 int nlh_get_tuning(nlh_handle *h, nlh_tuning *t);

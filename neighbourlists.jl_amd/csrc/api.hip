@@ -109,8 +109,8 @@ struct DevBuf {
 struct nlh_handle {
     int device = 0;
     hipStream_t stream = nullptr;
-    DevBuf X, tables, key, slot, cell_count, cell_start, tmp, rec, wbig, perm, cell_id,
-        cell_offsets, pair_count, inv_perm, slots, slotsR, pair_off, tiles, lookback, sysmap, sys_pair_base, first, i, j, S, R, status, Xsorted, nb_j, nb_R,
+    DevBuf X, tables, key, slot, cell_count, bmat, recb, cell_start, tmp, rec, wbig, perm, cell_id,
+        cell_offsets, pair_count, slots, slotsR, pair_off, tiles, lookback, sysmap, sys_pair_base, first, i, j, S, R, status, Xsorted, nb_j, nb_R,
         nb_S, nb_count;
     // system tables, one device buffer: SysDev[nsys], atom bases int64[nsys + 1],
     // cell bases int32[nsys + 1], brick bases int32[nsys + 1]
@@ -162,7 +162,7 @@ struct nlh_handle {
     nlh::PipelineBufs last_bufs;         // ... on these buffers and sizes
     int64_t graph_launches = 0;
     // launch configuration (nlh_set_tuning / nlh_autotune)
-    nlh_tuning tune = {768, 256, 256, 0};
+    nlh_tuning tune = {768, 256, 256, 0, 0};
 
     // count-pass workgroup: 4 waves per brick unless tuned (four 36 KB
     // bricks share a CU; 8-wave bricks of 72 KB fit two)
@@ -181,9 +181,18 @@ struct nlh_handle {
         b.nsys = nsys;
         b.nat = nat;
         b.M = M;
+        b.sort = nlh::sort_plan(M, nat, tune.cell_sort);
         b.key = key.as<int32_t>();
         b.slot = slot.as<int32_t>();
         b.cell_count = cell_count.as<int32_t>();
+        b.bmat = bmat.as<int32_t>();
+        // the bucketed sort's scratch: the slot rows once they are allocated
+        // (rewritten by every count pass), else a buffer of its own
+        {
+            char *sc = slots.cap >= (size_t)std::max<int64_t>(nat, 1) * nlh::kBucketBytes ? (char *)slots.p
+                                                                                          : (char *)recb.p;
+            b.bke = reinterpret_cast<int2 *>(sc);
+        }
         b.cell_start = cell_start.as<int32_t>();
         b.tmp = tmp.as<int32_t>();
         b.rec = rec.as<nlh::Rec>();
@@ -192,7 +201,6 @@ struct nlh_handle {
         b.cell_id_out = cell_id.p;
         b.cell_offsets_out = cell_offsets.p;
         b.pair_count = pair_count.as<int32_t>();
-        b.inv_perm = inv_perm.as<int32_t>();
         b.slots = slots.p;
         b.slotsR = slotsR.as<double>();
         b.slot_cap = slot_cap;
@@ -511,13 +519,21 @@ void setup_batch(nlh_handle *h, int32_t nsys, const int64_t *atom_offsets, const
     h->cell_count.ensure(h->stream, M1 * 4, /*zero=*/true);
     h->cell_start.ensure(h->stream, M1 * 4);
     h->tmp.ensure(h->stream, n1 * 4);
-    h->inv_perm.ensure(h->stream, n1 * 4);
+    // the bucketed sort's buffers whatever the tuning says (it may change
+    // before the next build)
+    const nlh::SortPlan sp = nlh::sort_plan(cells, nat, 0);
+    if (sp.bw >= 0) {
+        h->bmat.ensure(h->stream, (size_t)sp.n_sort * 4);
+        if (h->slots.cap < n1 * nlh::kBucketBytes) h->recb.ensure(h->stream, n1 * nlh::kBucketBytes);
+    }
     h->rec.ensure(h->stream, n1 * sizeof(nlh::Rec));
     h->wbig.ensure(h->stream, n1 * 12);
     h->perm.ensure(h->stream, n1 * ti);
     h->cell_id.ensure(h->stream, n1 * ti);
     h->cell_offsets.ensure(h->stream, (M1 + nsys) * ti);
-    h->lookback.ensure(h->stream, (size_t)nlh::lookback_entries((int64_t)M1, (int64_t)n1) * sizeof(uint64_t));
+    h->lookback.ensure(h->stream,
+                       (size_t)nlh::lookback_entries(std::max<int64_t>((int64_t)M1, sp.n_sort), (int64_t)n1) *
+                           sizeof(uint64_t));
     if (nsys > 1) h->sysmap.ensure(h->stream, (n1 + M1) * 4);   // system of each atom, then of each cell
     h->status.ensure(h->stream, sizeof(nlh::Status));
 
@@ -703,7 +719,7 @@ void check_pair_buffers(const nlh_handle *h, bool want_R)
     const size_t ent = (size_t)h->slot_cap * n1;
     const size_t ti = (size_t)h->ti_bytes, cap = (size_t)std::max<int64_t>(h->pair_capacity, 0);
     bool ok = h->slot_cap > 0 && h->slots.p && h->slots.cap >= ent * (h->wide ? 16 : 8);
-    ok = ok && h->pair_count.cap >= n1 * 4 && h->pair_off.cap >= (n1 + 1) * 8 && h->inv_perm.cap >= n1 * 4;
+    ok = ok && h->pair_count.cap >= n1 * 4 && h->pair_off.cap >= (n1 + 1) * 8;
     ok = ok && h->first.cap >= (n1 + h->nsys) * ti && h->sys_pair_base.cap >= (size_t)(h->nsys + 1) * 8;
     ok = ok && h->i.cap >= cap * ti && h->j.cap >= cap * ti && h->S.cap >= cap * 3 * ti;
     ok = ok && h->tiles.cap >= (cap / nlh::kMinFillPairs + 2) * 8;
@@ -868,6 +884,8 @@ void validate_tuning(const nlh_tuning &t)
         throw Error(NLH_EINVAL, "fill_pairs must be 128, 256 or 512");
     if (t.count_block != 0 && t.count_block != 256 && t.count_block != 512)
         throw Error(NLH_EINVAL, "count_block must be 0 (automatic), 256 or 512");
+    if (t.cell_sort != 0 && t.cell_sort != 1)
+        throw Error(NLH_EINVAL, "cell_sort must be 0 (bucketed where it applies) or 1 (atomic histogram)");
 }
 
 void apply_tuning(nlh_handle *h, const nlh_tuning &t)
@@ -999,8 +1017,8 @@ int nlh_destroy(nlh_handle *h)
                 if (e) (void)hipEventDestroy(e);
         });
         DevBuf *bufs[] = {&h->X,        &h->tables,    &h->key,    &h->slot,
-                          &h->cell_count, &h->cell_start, &h->tmp,      &h->rec,          &h->wbig,   &h->perm,
-                          &h->cell_id,  &h->cell_offsets, &h->pair_count, &h->inv_perm, &h->slots, &h->slotsR, &h->pair_off,   &h->tiles, &h->lookback, &h->sysmap,
+                          &h->cell_count, &h->bmat,     &h->recb,       &h->cell_start, &h->tmp,    &h->rec,
+                          &h->wbig,     &h->perm,       &h->cell_id,  &h->cell_offsets, &h->pair_count, &h->slots, &h->slotsR, &h->pair_off,   &h->tiles, &h->lookback, &h->sysmap,
                           &h->sys_pair_base, &h->first, &h->i,          &h->j,            &h->S,      &h->R,
                           &h->status,   &h->Xsorted,   &h->nb_j,       &h->nb_R,         &h->nb_S,   &h->nb_count};
         for (DevBuf *b : bufs) b->release();
@@ -1343,7 +1361,7 @@ int nlh_autotune(nlh_handle *h, const double *X_batch, uint32_t flags, int32_t r
                 for (int32_t fb : fills)
                     for (int32_t fa : tiles) {
                     if (cb == 256 && ba > 768) continue;   // above what a 256-thread brick holds
-                    apply_tuning(h, nlh_tuning{ba, fb, fa, cb});
+                    apply_tuning(h, nlh_tuning{ba, fb, fa, cb, start.cell_sort});
                     rebuild_eager(h, flags & ~NLH_NO_SYNC);   // warm-up (and buffer growth)
                     float tot = 0.0f;
                     for (int r = 0; r < reps; ++r) {

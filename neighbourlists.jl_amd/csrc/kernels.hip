@@ -1,14 +1,22 @@
 // nlhip device kernels for the sort-based neighbour-list build (gfx950, wave64).
 //
-// Pipeline (a single build is a batch of one system):
-//   bin_kernel        cell id + winding per atom, cell histogram by atomics
-//                     (compute_cell_ids_kernel!, gpu_kernels.jl:108-125, and
-//                     _histogram_kernel!, :191-197)
-//   scan (cells)      cell_start / cell_offsets (_compute_cell_offsets,
-//                     gpu_kernels.jl:262-285)
-//   scatter_kernel    group atom indices by cell (arrival order)
-//   place_kernel      stable in-cell rank -> perm, cell_id, sorted records
-//                     (sortperm + gathers, cell_list.jl:668-670)
+// Pipeline (a single build is a batch of one system).  The cell sort
+// (compute_cell_ids_kernel!, gpu_kernels.jl:108-125; the histogram and
+// offsets, :191-197 and :262-285; sortperm + gathers, cell_list.jl:668-670)
+// has two forms with the same result:
+//   bucketed (default)
+//     bucket_count_kernel   cell id + winding per atom, counts per bucket of
+//                           cells in LDS, per block of atoms
+//     scan (counts)         every block's place in every bucket
+//     bucket_scatter_kernel atoms grouped by bucket
+//     bucket_place_kernel   one workgroup per bucket: cell_start / cell_offsets,
+//                           stable in-cell rank -> perm, cell_id, sorted records
+//   atomic
+//     bin_kernel            cell id + winding, cell histogram by global atomics
+//     scan (cells)          cell_start / cell_offsets
+//     scatter_kernel        group atom indices by cell (arrival order)
+//     place_kernel          stable in-cell rank -> perm, cell_id, sorted records
+// then
 //   scan (pairs)      first / pair offsets (compute_pair_offsets,
 //                     gpu_kernels.jl:222-237)
 // The pair search itself (count and fill) is in pairs.hip.
@@ -384,8 +392,8 @@ __global__ __launch_bounds__(256) void place_kernel(const double *__restrict__ X
                                                     const int32_t *__restrict__ cell_sys, int nsys,
                                                     const int32_t *__restrict__ key, const int32_t *__restrict__ cell_start,
                                                     const int32_t *__restrict__ tmp, int64_t nat, Rec *__restrict__ rec,
-                                                    int32_t *__restrict__ wbig, int32_t *__restrict__ inv_perm,
-                                                    void *__restrict__ perm_out, void *__restrict__ cell_id_out)
+                                                    int32_t *__restrict__ wbig, void *__restrict__ perm_out,
+                                                    void *__restrict__ cell_id_out)
 {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nat) return;
@@ -405,7 +413,6 @@ __global__ __launch_bounds__(256) void place_kernel(const double *__restrict__ X
     r.x = x; r.y = y; r.z = z; r.j = j;
     r.w = pack_wind(w[0], w[1], w[2]);
     rec[p] = r;
-    inv_perm[e] = p;
     if (r.w & kWindBig) {
         wbig[3 * (int64_t)p] = w[0];
         wbig[3 * (int64_t)p + 1] = w[1];
@@ -426,6 +433,259 @@ __global__ __launch_bounds__(256) void sorted_positions_kernel(const Rec *__rest
     Xs[3 * p + 2] = r.z;
 }
 
+// ---------------------------------------------------------------------------
+// 3b. bucketed cell sort (the default form; see the top of the file)
+// ---------------------------------------------------------------------------
+//
+// The global-atomic histogram costs one L2 atomic per atom, and those
+// atomics run at a fixed rate (1M atoms: 38 of bin_kernel's 44 us).  Here
+// every counter lives in LDS: a block of P.apb consecutive atoms counts them
+// per bucket (2^P.bw consecutive cells), one scan over the bucket-major
+// P.nb x P.na counts places every (bucket, block) run, and one workgroup per
+// bucket orders its few hundred to few thousand atoms by cell and original
+// index with LDS counters again.  Only (cell, index) pairs move through the
+// scatter; the ordering pass reads the positions from X itself.
+
+__global__ __launch_bounds__(kBinThreads) void bucket_count_kernel(
+    const double *__restrict__ X, const SysDev *__restrict__ sys, const int32_t *__restrict__ atom_sys, int nsys,
+    int64_t nat, SortPlan P, int32_t *__restrict__ key, int32_t *__restrict__ slot, int32_t *__restrict__ bmat,
+    Status *__restrict__ status)
+{
+    extern __shared__ int32_t hist[];   // P.nb counters
+    for (int b = threadIdx.x; b < P.nb; b += kBinThreads) hist[b] = 0;
+    __syncthreads();
+    const int64_t a0 = (int64_t)blockIdx.x * P.apb;
+    const int64_t a1 = a0 + P.apb < nat ? a0 + P.apb : nat;
+    bool bad = false;
+    for (int64_t g0 = a0 + threadIdx.x; g0 < a1; g0 += kBinThreads * kBinUnroll) {
+        double x[kBinUnroll], y[kBinUnroll], z[kBinUnroll];
+        int sy[kBinUnroll];
+#pragma unroll
+        for (int u = 0; u < kBinUnroll; ++u) {   // all loads of the thread in flight together
+            const int64_t g = g0 + (int64_t)u * kBinThreads;
+            x[u] = y[u] = z[u] = 0.0;
+            sy[u] = 0;
+            if (g < a1) {
+                x[u] = X[3 * g];
+                y[u] = X[3 * g + 1];
+                z[u] = X[3 * g + 2];
+                if (nsys != 1) sy[u] = atom_sys[g];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kBinUnroll; ++u) {
+            const int64_t g = g0 + (int64_t)u * kBinThreads;
+            if (g < a1) {
+                const SysDev &sd = sys[sy[u]];
+                int32_t c[3], w[3];
+                bad |= !cell_and_winding(sd, x[u], y[u], z[u], c, w);
+                const int32_t k = sd.cell_base + (c[0] - 1) + (c[1] - 1) * sd.n[0] + (c[2] - 1) * sd.n[0] * sd.n[1];
+                // read once more, by the scatter: streamed past the caches like the
+                // sort's other whole-line scratch traffic (the pair pass later relies
+                // on the last-level cache holding its slot rows)
+                __builtin_nontemporal_store(k, key + g);
+                __builtin_nontemporal_store(atomicAdd(&hist[k >> P.bw], 1), slot + g);   // arrival rank in (block, bucket)
+            }
+        }
+    }
+    if (bad) atomicOr(&status->bad_position, 1);
+    __syncthreads();
+    for (int b = threadIdx.x; b < P.nb; b += kBinThreads) bmat[(int64_t)b * P.na + blockIdx.x] = hist[b];
+}
+
+// Exclusive scan of n counts in place (the bucket counts); tile prefixes
+// by look-back, or from the tile sums of the three-kernel form.
+__global__ __launch_bounds__(kScanThreads) void scan_inplace_kernel(int32_t *__restrict__ a, int64_t n,
+                                                                    uint64_t *__restrict__ desc,
+                                                                    const int64_t *__restrict__ partials,
+                                                                    Status *__restrict__ status)
+{
+    const int64_t tile = scan_tile_begin(desc, partials, status);
+    if (tile < 0) return;
+    int32_t v[kScanItems];
+    const int64_t base = tile * kScanTile;
+    load_tile(a, n, base, v);
+    int64_t s = 0;
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q) s += v[q];
+    int64_t tot;
+    const int64_t ex = block_excl_scan(s, tot);
+    int64_t run = ex + (partials ? partials[tile] : scan_lookback(desc, tile, tot, status));
+    const int64_t b = base + (int64_t)threadIdx.x * kScanItems;
+    int32_t o[kScanItems];
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q) {
+        o[q] = (int32_t)run;
+        run += v[q];
+    }
+    if (b + kScanItems <= n) {
+        int4 *p = reinterpret_cast<int4 *>(a + b);
+#pragma unroll
+        for (int q = 0; q < kScanItems / 4; ++q) p[q] = make_int4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < kScanItems; ++q)
+            if (b + q < n) a[b + q] = o[q];
+    }
+}
+
+__global__ __launch_bounds__(kBinThreads) void bucket_scatter_kernel(int64_t nat, SortPlan P,
+                                                                     const int32_t *__restrict__ key,
+                                                                     const int32_t *__restrict__ slot,
+                                                                     const int32_t *__restrict__ boff,
+                                                                     int2 *__restrict__ bke)
+{
+    const int64_t a0 = (int64_t)blockIdx.x * P.apb;
+    const int64_t a1 = a0 + P.apb < nat ? a0 + P.apb : nat;
+    for (int64_t g0 = a0 + threadIdx.x; g0 < a1; g0 += kBinThreads * kBinUnroll) {
+        int32_t k[kBinUnroll], pos[kBinUnroll];
+#pragma unroll
+        for (int u = 0; u < kBinUnroll; ++u) {
+            const int64_t g = g0 + (int64_t)u * kBinThreads;
+            k[u] = 0;
+            pos[u] = 0;
+            if (g < a1) {
+                k[u] = __builtin_nontemporal_load(key + g);
+                pos[u] = __builtin_nontemporal_load(slot + g);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kBinUnroll; ++u) {
+            const int64_t g = g0 + (int64_t)u * kBinThreads;
+            // scattered stores: cached, so that the L2 merges them into lines
+            if (g < a1) bke[boff[(int64_t)(k[u] >> P.bw) * P.na + blockIdx.x] + pos[u]] = make_int2(k[u], (int32_t)g);
+        }
+    }
+}
+
+__device__ __forceinline__ int2 load_ke(const int2 *p)
+{
+    const uint64_t v = __builtin_nontemporal_load(reinterpret_cast<const uint64_t *>(p));
+    return make_int2((int32_t)(uint32_t)v, (int32_t)(uint32_t)(v >> 32));
+}
+
+// One workgroup per bucket: its cells' counts and starts in LDS, the atoms
+// grouped by cell in arrival order, then each atom's rank among its cell's
+// atoms by original index -- ties of the cell key keep ascending original
+// index, the order of Julia's stable sortperm.  A bucket of at most
+// kBucketLdsAtoms atoms keeps its cells, indices and groups in LDS; a larger
+// one (clustered atoms) works from global memory (bke and tmp, its own range).
+template <typename TI>
+__global__ __launch_bounds__(kBucketThreads) void bucket_place_kernel(
+    const int2 *__restrict__ bke, const double *__restrict__ X, const int32_t *__restrict__ boff, SortPlan P,
+    int64_t nat, int32_t M, const SysDev *__restrict__ sys, const int32_t *__restrict__ cell_sys, int nsys,
+    int32_t *__restrict__ tmp, int32_t *__restrict__ cell_start, void *__restrict__ offsets_out,
+    Rec *__restrict__ rec, int32_t *__restrict__ wbig, void *__restrict__ perm_out, void *__restrict__ cell_id_out)
+{
+    __shared__ int32_t kl[kBucketLdsAtoms], el[kBucketLdsAtoms], grp[kBucketLdsAtoms];
+    extern __shared__ int32_t lds[];
+    const int b = (int)blockIdx.x;
+    const int32_t c0 = b << P.bw;
+    const int W = (1 << P.bw) < M - c0 ? (1 << P.bw) : M - c0;
+    int32_t *co = lds;                        // W + 1: cell starts within the bucket
+    int32_t *fill = lds + (1 << P.bw) + 1;    // W: arrivals per cell
+    const int32_t bs = boff[(int64_t)b * P.na];
+    const int32_t be = b + 1 < P.nb ? boff[(int64_t)(b + 1) * P.na] : (int32_t)nat;
+    const int n = be - bs;
+    const bool small = n <= kBucketLdsAtoms;   // uniform
+    for (int t = threadIdx.x; t < W; t += kBucketThreads) {
+        co[t] = 0;
+        fill[t] = 0;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += kBucketThreads) {
+        const int2 ke = load_ke(bke + bs + i);
+        if (small) {
+            kl[i] = ke.x;
+            el[i] = ke.y;
+        }
+        atomicAdd(&co[ke.x - c0], 1);
+    }
+    __syncthreads();
+    // exclusive scan of co[0, W): thread t takes a run of `per` cells
+    const int per = (W + kBucketThreads - 1) / kBucketThreads;
+    const int t0 = (int)threadIdx.x * per < W ? (int)threadIdx.x * per : W;
+    const int t1 = t0 + per < W ? t0 + per : W;
+    int32_t sum = 0;
+    for (int t = t0; t < t1; ++t) sum += co[t];
+    int64_t tot;
+    int32_t run = (int32_t)block_excl_scan_n<kBucketThreads>(sum, tot);   // its barriers end every read of co
+    for (int t = t0; t < t1; ++t) {
+        const int32_t v = co[t];
+        co[t] = run;
+        run += v;
+    }
+    if (threadIdx.x == 0) co[W] = (int32_t)tot;
+    __syncthreads();
+    for (int t = threadIdx.x; t < W; t += kBucketThreads) {
+        const int32_t c = c0 + t;
+        const int32_t r0 = bs + co[t];
+        cell_start[c] = r0;
+        const int si = nsys == 1 ? 0 : cell_sys[c];
+        const SysDev &sd = sys[si];
+        store_ti<TI>(offsets_out, (int64_t)c + si, r0 - sd.atom_base + 1);
+        if (c == sd.cell_base + sd.M - 1)   // trailing entry of this system
+            store_ti<TI>(offsets_out, (int64_t)c + si + 1, bs + co[t + 1] - sd.atom_base + 1);
+    }
+    if (b == P.nb - 1 && threadIdx.x == 0) cell_start[M] = (int32_t)nat;
+    int32_t *grpg = tmp + bs;   // the large-bucket groups
+    for (int i = threadIdx.x; i < n; i += kBucketThreads) {
+        int32_t k, e;
+        if (small) {
+            k = kl[i];
+            e = el[i];
+        } else {
+            const int2 ke = load_ke(bke + bs + i);
+            k = ke.x;
+            e = ke.y;
+        }
+        const int c = k - c0;
+        const int q = co[c] + atomicAdd(&fill[c], 1);
+        if (small) grp[q] = e;
+        else grpg[q] = e;
+    }
+    __syncthreads();   // the groups (LDS, or the block's own tmp stores) are visible to the block
+    for (int i = threadIdx.x; i < n; i += kBucketThreads) {
+        int32_t k, e;
+        if (small) {
+            k = kl[i];
+            e = el[i];
+        } else {
+            const int2 ke = load_ke(bke + bs + i);
+            k = ke.x;
+            e = ke.y;
+        }
+        const int c = k - c0;
+        const int s0 = co[c], s1 = co[c + 1];
+        int32_t rank = 0;
+        if (small)
+            for (int t = s0; t < s1; ++t) rank += grp[t] < e;
+        else
+            for (int t = s0; t < s1; ++t) rank += grpg[t] < e;
+        const int32_t p = bs + s0 + rank;
+        const int si = nsys == 1 ? 0 : cell_sys[k];
+        const SysDev &sd = sys[si];
+        const double x = X[3 * (int64_t)e], y = X[3 * (int64_t)e + 1], z = X[3 * (int64_t)e + 2];
+        int32_t cc[3], w[3];
+        cell_and_winding(sd, x, y, z, cc, w);
+        const int32_t j = (int32_t)(e - sd.atom_base);
+        Rec o;
+        o.x = x;
+        o.y = y;
+        o.z = z;
+        o.j = j;
+        o.w = pack_wind(w[0], w[1], w[2]);
+        rec[p] = o;
+        if (o.w & kWindBig) {
+            wbig[3 * (int64_t)p] = w[0];
+            wbig[3 * (int64_t)p + 1] = w[1];
+            wbig[3 * (int64_t)p + 2] = w[2];
+        }
+        store_ti<TI>(perm_out, p, (int64_t)j + 1);
+        store_ti<TI>(cell_id_out, p, (int64_t)(k - sd.cell_base) + 1);
+    }
+}
+
 // Zero the build status (when given) and n scan descriptors.  A kernel
 // rather than a memset, so the pipeline can be captured into a HIP graph and
 // replayed.
@@ -444,7 +704,7 @@ static inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t -
 
 void launch_clear_status(const PipelineBufs &b, hipStream_t st)
 {
-    const int64_t n = lookback_entries(b.M, b.nat);
+    const int64_t n = lookback_entries(b.sort.n_sort, b.nat);
     hipLaunchKernelGGL(clear_status_kernel, dim3(std::min(blocks_for(n, 256), 64u)), dim3(256), 0, st, b.status,
                        b.lookback, n);
 }
@@ -453,7 +713,7 @@ void launch_clear_pair_lookback(const PipelineBufs &b, hipStream_t st)
 {
     const int64_t n = scan_tiles(b.nat) + 1;
     hipLaunchKernelGGL(clear_status_kernel, dim3(std::min(blocks_for(n, 256), 64u)), dim3(256), 0, st,
-                       (Status *)nullptr, b.lookback + lookback_cells(b.M), n);
+                       (Status *)nullptr, b.lookback + lookback_cells(b.sort.n_sort), n);
 }
 
 #define NLH_TI_DISPATCH(ti, ...)            \
@@ -467,11 +727,41 @@ void launch_clear_pair_lookback(const PipelineBufs &b, hipStream_t st)
         }                                   \
     } while (0)
 
+static void launch_bucket_sort(const PipelineBufs &b, hipStream_t st)
+{
+    const SortPlan &P = b.sort;
+    hipLaunchKernelGGL(bucket_count_kernel, dim3((unsigned)P.na), dim3(kBinThreads), (size_t)P.nb * 4, st, b.X, b.sys,
+                       b.atom_sys, b.nsys, b.nat, P, b.key, b.slot, b.bmat, b.status);
+    const int64_t nt = scan_tiles(P.n_sort);
+    int64_t *part = nullptr;   // three-kernel scan: tile sums in the (then unused) descriptor slots
+    if (nt > kSortLookbackTiles) {
+        part = reinterpret_cast<int64_t *>(b.lookback);
+        hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)nt), dim3(kScanThreads), 0, st, b.bmat, P.n_sort, part,
+                           (int32_t)-1);
+        hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(kScanThreads), 0, st, part, (int)nt, (Status *)nullptr,
+                           (int64_t)0);
+    }
+    hipLaunchKernelGGL(scan_inplace_kernel, dim3((unsigned)nt), dim3(kScanThreads), 0, st, b.bmat, P.n_sort,
+                       b.lookback, part, b.status);
+    hipLaunchKernelGGL(bucket_scatter_kernel, dim3((unsigned)P.na), dim3(kBinThreads), 0, st, b.nat, P, b.key, b.slot,
+                       b.bmat, b.bke);
+    NLH_TI_DISPATCH(b.ti_bytes,
+                    hipLaunchKernelGGL(bucket_place_kernel<TI>, dim3((unsigned)P.nb), dim3(kBucketThreads),
+                                       (size_t)(2 * (1 << P.bw) + 1) * 4, st, b.bke, b.X, b.bmat, P, b.nat, b.M,
+                                       b.sys, b.nsys > 1 ? b.atom_sys + b.nat : nullptr, b.nsys, b.tmp,
+                                       b.cell_start, b.cell_offsets_out, b.rec, b.wbig, b.perm_out,
+                                       b.cell_id_out));
+}
+
 void launch_build(const PipelineBufs &b, hipStream_t st)
 {
     if (b.nsys > 1)
         hipLaunchKernelGGL(sys_map_kernel, dim3(blocks_for(b.nat + b.M, 256)), dim3(256), 0, st, b.sys_atom_base,
                            b.sys_cell_base, b.nsys, b.nat, (int64_t)b.M, b.atom_sys);
+    if (b.sort.bw >= 0) {
+        launch_bucket_sort(b, st);
+        return;
+    }
     if (b.nat > 0)
         hipLaunchKernelGGL(bin_kernel, dim3(blocks_for(b.nat, 256)), dim3(256), 0, st, b.X, b.sys, b.atom_sys,
                            b.nsys, b.nat, b.key, b.slot, b.cell_count, b.status);
@@ -496,7 +786,7 @@ void launch_build(const PipelineBufs &b, hipStream_t st)
                         hipLaunchKernelGGL(place_kernel<TI>, dim3(blocks_for(b.nat, 256)), dim3(256), 0, st, b.X,
                                            b.sys, b.nsys > 1 ? b.atom_sys + b.nat : nullptr, b.nsys, b.key,
                                            b.cell_start, b.tmp, b.nat, b.rec,
-                                           b.wbig, b.inv_perm, b.perm_out, b.cell_id_out));
+                                           b.wbig, b.perm_out, b.cell_id_out));
     }
 }
 
@@ -504,7 +794,7 @@ void launch_scan_pairs(const PipelineBufs &b, hipStream_t st)
 {
     const int64_t n = b.nat;
     const int64_t nbp = scan_tiles(n);
-    uint64_t *desc = b.lookback + lookback_cells(b.M);
+    uint64_t *desc = b.lookback + lookback_cells(b.sort.n_sort);
     int64_t *ppart = nullptr;   // three-kernel scan: tile sums in the (then unused) descriptor slots
     if (nbp > kLookbackTiles) {
         ppart = reinterpret_cast<int64_t *>(desc);

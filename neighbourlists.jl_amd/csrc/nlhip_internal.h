@@ -22,11 +22,58 @@ constexpr int kScanTile = kScanThreads * kScanItems;
 constexpr int kMinFillPairs = 128;        // smallest fill tile (pairs per fill wave)
 
 // Single-pass scan descriptors (a ticket plus one per tile of kScanTile
-// items): the cell scan's, then the pair scan's.
+// items): the cell sort's scan (n_sort items), then the pair scan's.
 inline int64_t scan_tiles(int64_t n) { return n > 0 ? (n + kScanTile - 1) / kScanTile : 1; }
 constexpr int64_t kLookbackTiles = 64;   // single-pass scans up to this many tiles
-inline int64_t lookback_cells(int64_t M) { return scan_tiles(M) + 1; }
-inline int64_t lookback_entries(int64_t M, int64_t nat) { return lookback_cells(M) + scan_tiles(nat) + 1; }
+constexpr int64_t kSortLookbackTiles = 128;   // the bucketed sort's count scan: single pass up to here
+inline int64_t lookback_cells(int64_t n_sort) { return scan_tiles(n_sort) + 1; }
+inline int64_t lookback_entries(int64_t n_sort, int64_t nat) { return lookback_cells(n_sort) + scan_tiles(nat) + 1; }
+
+// Cell sort.  Bucketed (the default): buckets are runs of 2^bw consecutive
+// global cells.  Blocks of apb consecutive atoms count their atoms per
+// bucket in LDS (na blocks, nb buckets), one scan over the nb x na counts
+// (bucket-major) gives every block its place in every bucket, the atoms are
+// scattered there, and one workgroup per bucket orders its atoms by (cell,
+// original index).  Atomic (cell_sort = 1, or grids beyond the bucket
+// limits): one global atomic histogram over the cells, scan, grouping and a
+// stable in-cell rank.  Both give the same result.
+constexpr int kBinThreads = 512;          // threads of the bucket count / scatter blocks
+constexpr int kBinUnroll = 4;             // atoms a thread loads before it works on them
+constexpr int kBucketThreads = 512;       // threads of a bucket's sorting workgroup
+constexpr int kBucketLdsAtoms = 4096;     // buckets up to this size are ordered in LDS
+constexpr int kBucketMaxShift = 12;       // at most 4096 cells per bucket
+constexpr int kBucketMaxCount = 8192;     // buckets (LDS counters of a count block)
+constexpr int kBucketBytes = 8;           // scratch per atom: (cell, index)
+struct SortPlan {
+    int32_t bw;     // log2 cells per bucket; -1: atomic histogram sort
+    int32_t nb;     // buckets
+    int32_t na;     // count / scatter blocks
+    int32_t apb;    // atoms per count / scatter block (a multiple of kBinThreads)
+    int64_t n_sort; // items of the sort's scan: nb * na, or M for the atomic sort
+};
+inline SortPlan sort_plan(int64_t M, int64_t nat, int cell_sort)
+{
+    SortPlan p{-1, 0, 0, 0, M};
+    if (cell_sort != 0 || nat <= 0 || M <= 0) return p;
+    // at least 256 buckets (workgroups of the ordering pass), about 1600
+    // atoms each from 400k atoms on, at most 1024
+    const int64_t target = nat / 1600 < 256 ? 256 : (nat / 1600 > 1024 ? 1024 : nat / 1600);
+    int bw = 0;
+    while (bw < kBucketMaxShift && ((M + (1ll << bw) - 1) >> bw) > target) ++bw;
+    const int64_t nb = (M + (1ll << bw) - 1) >> bw;
+    if (nb > kBucketMaxCount) return p;
+    // as many count blocks (up to 512) as keep nb x na within a single-pass scan
+    int64_t na = kSortLookbackTiles * kScanTile / nb;
+    na = na < 1 ? 1 : (na > 512 ? 512 : na);
+    int64_t apb = (nat + na - 1) / na;
+    apb = (apb + kBinThreads - 1) / kBinThreads * kBinThreads;
+    p.bw = bw;
+    p.nb = (int32_t)nb;
+    p.apb = (int32_t)apb;
+    p.na = (int32_t)((nat + apb - 1) / apb);
+    p.n_sort = nb * p.na;
+    return p;
+}
 
 // log2 of the fill tile (pairs per fill wave: 128, 256 or 512)
 inline int fill_tile_shift(int fill_pairs) { return fill_pairs == 128 ? 7 : (fill_pairs == 512 ? 9 : 8); }
@@ -57,6 +104,7 @@ struct __attribute__((aligned(16))) Rec {
 };
 
 constexpr uint32_t kWindBig = 0x80000000u;
+
 
 // Per-atom neighbour counts carry the format of the atom's slot row in bit
 // 30: clear = 4-byte entries (j | S code << 26, every |S_k| <= 1 and j < 2^26),
@@ -183,9 +231,12 @@ struct PipelineBufs {
     int64_t nat;                // total atoms
     int32_t M;                  // total cells
     // build
+    SortPlan sort;
     int32_t *key;               // nat: global cell id
-    int32_t *slot;              // nat: arrival rank from the histogram atomics
-    int32_t *cell_count;        // M + 1, zero between builds
+    int32_t *slot;              // nat: arrival rank (histogram atomics, or the count block's LDS counters)
+    int32_t *cell_count;        // M + 1, zero between builds (atomic sort)
+    int32_t *bmat;              // nb * na bucket counts, then their exclusive scan (bucketed sort)
+    int2 *bke;                  // nat: (global cell, global index) grouped by bucket (bucketed sort)
     int32_t *cell_start;        // M + 1
     int32_t *tmp;               // nat: cell-grouped, not yet stable
     Rec *rec;                   // nat: sorted records
@@ -195,7 +246,6 @@ struct PipelineBufs {
     void *cell_offsets_out;     // M + nsys, TI, 1-based per system
     // pairs
     int32_t *pair_count;
-    int32_t *inv_perm;          // nat: original -> sorted index
     void *slots;                // nat * slot_cap slot entries
     double *slotsR;             // 3 * nat * slot_cap (R requested)
     int32_t slot_cap;

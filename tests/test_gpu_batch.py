@@ -98,16 +98,16 @@ def test_graph_replay_outgrows_capacity():
     _check_oracle(plain.rebuild(Xd_), dense, Xd_, offs, range(24), R=False)
 
 
-@pytest.mark.parametrize("brick_atoms,fill_block,fill_pairs,count_block",
-                         [(64, 64, 128, 0), (512, 128, 512, 256), (1024, 512, 256, 512), (2048, 1024, 128, 256),
-                          (1536, 256, 512, 512), (768, 256, 256, 256)])
-def test_tuning_does_not_change_results(brick_atoms, fill_block, fill_pairs, count_block):
+@pytest.mark.parametrize("brick_atoms,fill_block,fill_pairs,count_block,cell_sort",
+                         [(64, 64, 128, 0, 1), (512, 128, 512, 256, 0), (1024, 512, 256, 512, 1),
+                          (2048, 1024, 128, 256, 0), (1536, 256, 512, 512, 1), (768, 256, 256, 256, 0)])
+def test_tuning_does_not_change_results(brick_atoms, fill_block, fill_pairs, count_block, cell_sort):
     s = workloads.random_cubic(6000, density=0.05, seed=81)
     h = _lib.Handle()
-    h.set_tuning(brick_atoms, fill_block, fill_pairs, count_block)
+    h.set_tuning(brick_atoms, fill_block, fill_pairs, count_block, cell_sort)
     t = h.get_tuning()
-    assert (t.brick_atoms, t.fill_block, t.fill_pairs, t.count_block) == (brick_atoms, fill_block, fill_pairs,
-                                                                          count_block)
+    assert (t.brick_atoms, t.fill_block, t.fill_pairs, t.count_block, t.cell_sort) == (
+        brick_atoms, fill_block, fill_pairs, count_block, cell_sort)
     nl = nlhip.neighbour_list(s.X, 5.0, s.cell, (True, False, True), want_R=True, handle=h)
     ref = oracle.neighbour_list(s.X, 5.0, s.cell, (True, False, True), want_R=True)
     _same(nl, ref)
@@ -123,6 +123,8 @@ def test_tuning_rejects_bad_values():
         h.set_tuning(1024, 256, 200)
     with pytest.raises(_lib.NLHInvalidError):
         h.set_tuning(1024, 256, 256, 128)
+    with pytest.raises(_lib.NLHInvalidError):
+        h.set_tuning(1024, 256, 256, 256, 2)
 
 
 def test_autotune_keeps_results():

@@ -161,6 +161,39 @@ def test_sorted_cell_list_fields():
         assert nlhip.num_neighbours(nl, i) == b - a
 
 
+@pytest.mark.parametrize("cell_sort", [0, 1])
+def test_cell_sort_forms(cell_sort):
+    """Both forms of the cell sort (buckets counted in LDS, or one global
+    atomic histogram) give the reference's cell list and pairs: a clustered
+    system (most atoms in a few cells of each bucket), a sparse grid (many
+    empty buckets) and a batch whose buckets straddle system boundaries."""
+    from nlhip import _lib
+    from nlhip.batch import BatchBuilder, pack_systems
+
+    rng = np.random.default_rng(95)
+    L = 40.0
+    Xc = np.concatenate([rng.random((3000, 3)) * 6.0 + 10.0, rng.random((1000, 3)) * L])
+    Xs = rng.random((700, 3)) * 400.0
+    for X, C, rc, pbc in [(Xc, np.eye(3) * L, 4.0, (True, True, True)),
+                          (Xs, np.eye(3) * 400.0, 3.0, (True, False, True))]:
+        h = _lib.Handle()
+        h.set_tuning(768, 256, 256, 0, cell_sort)
+        cl = nlhip.build_cell_list(X, rc, C, pbc, handle=h)
+        ref = oracle.neighbour_list(X, rc, C, pbc, want_R=True)
+        assert np.array_equal(cl.perm, ref.perm)
+        assert np.array_equal(cl.cell_id, ref.cell_id)
+        assert np.array_equal(cl.cell_offsets, ref.cell_offsets)
+        _same(nlhip.materialize_pairlist(cl, want_R=True), ref)
+    systems = workloads.batch_systems(40, seed=96, nmax=500)
+    X0, offs = pack_systems(systems)
+    bb = BatchBuilder()
+    bb.handle.set_tuning(768, 256, 256, 0, cell_sort)
+    res = bb.run(systems, want_R=True)
+    for s in (0, 1, 17, 39):
+        sysm = systems[s]
+        _same(res.pairlist(s), oracle.neighbour_list(sysm.X, sysm.cutoff, sysm.cell, sysm.pbc, want_R=True))
+
+
 def test_batch_matches_single():
     systems = workloads.batch_systems(64, seed=3, nmax=400)
     res = nlhip.batch_neighbour_list(systems, want_R=True)
