@@ -77,28 +77,36 @@ struct DevBuf {
     template <typename T>
     T *as() const { return static_cast<T *>(p); }
     // returns true when (re)allocated
-    // `st` is the owning handle's stream, the only stream that uses the buffer
+    // `st` is the owning handle's stream, the only stream that uses the buffer.
+    // Blocks come from the device's stream-ordered pool: the old block is
+    // freed after the owner's work on it and the new one is allocated in the
+    // same stream, so a growth waits for this stream only -- a plain free
+    // would wait for every stream on the device (with several handles
+    // building at once, e.g. the serving workers, each growth stalled all of
+    // them).
     bool ensure(hipStream_t st, size_t bytes, bool zero = false)
     {
         if (bytes == 0) bytes = 16;
         if (bytes <= cap) return false;
-        // growth is rare: the old block goes once the owner's work on it is done
-        HIPC(hipStreamSynchronize(st));
-        if (p) HIPD(hipFree(p));
-        p = nullptr;
         // a quarter of headroom: builds of varying size regrow rarely
         const size_t nb = std::max(bytes + bytes / 4, (size_t)256);
-        HIPD(hipMalloc(&p, nb));
+        device_wide([&] {
+            if (p) HIPD(hipFreeAsync(p, st));
+            p = nullptr;
+            cap = 0;
+            HIPD(hipMallocAsync(&p, nb, st));
+        });
         cap = nb;
-        if (zero) {
-            HIPC(hipMemsetAsync(p, 0, nb, st));
-            HIPC(hipStreamSynchronize(st));
-        }
+        if (zero) HIPC(hipMemsetAsync(p, 0, nb, st));
+        HIPC(hipStreamSynchronize(st));   // the block is ready for the host and any stream
         return true;
     }
-    void release()
+    void release(hipStream_t st)
     {
-        if (p) device_wide([&] { (void)hipFree(p); });
+        if (p) device_wide([&] {
+            (void)hipFreeAsync(p, st);
+            (void)hipStreamSynchronize(st);
+        });
         p = nullptr;
         cap = 0;
     }
@@ -771,7 +779,7 @@ void materialize(nlh_handle *h, uint32_t flags)
         // (the counts, offsets and totals of the first pass stay valid)
         if (st.wide) h->wide = true;
         ensure_pair_outputs(h, std::max<int64_t>(st.npairs, 1), want_R);
-        h->slots.release();
+        h->slots.release(h->stream);
         ensure_slots(h, round_slot_cap(std::max(st.max_count, 1)), want_R);
         check_pair_buffers(h, want_R);
         b = h->bufs();
@@ -970,6 +978,16 @@ int nlh_version(int32_t *major, int32_t *minor)
 
 const char *nlh_last_error(void) { return g_last_error.c_str(); }
 
+// Handle buffers come from the device's stream-ordered pool; keep freed
+// blocks in it for the next growth rather than returning them at every sync.
+static void keep_pool_blocks(int device)
+{
+    hipMemPool_t pool = nullptr;
+    HIPC(hipDeviceGetDefaultMemPool(&pool, device));
+    uint64_t keep = UINT64_MAX;
+    HIPC(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+}
+
 int nlh_init(int device)
 {
     return guarded([&] {
@@ -978,6 +996,7 @@ int nlh_init(int device)
         if (device < 0 || device >= n) throw Error(NLH_EINVAL, "no such device");
         HIPC(hipSetDevice(device));
         g_device = device;
+        keep_pool_blocks(device);
     });
 }
 
@@ -992,6 +1011,7 @@ int nlh_create(nlh_handle **out)
         h->device = g_device;
         try {
             HIPC(hipSetDevice(h->device));
+            keep_pool_blocks(h->device);
             HIPD(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
             HIPD(hipHostMalloc((void **)&h->status_host, sizeof(nlh::Status),
                                hipHostMallocMapped | hipHostMallocCoherent));
@@ -1021,7 +1041,7 @@ int nlh_destroy(nlh_handle *h)
                           &h->wbig,     &h->perm,       &h->cell_id,  &h->cell_offsets, &h->pair_count, &h->slots, &h->slotsR, &h->pair_off,   &h->tiles, &h->lookback, &h->sysmap,
                           &h->sys_pair_base, &h->first, &h->i,          &h->j,            &h->S,      &h->R,
                           &h->status,   &h->Xsorted,   &h->nb_j,       &h->nb_R,         &h->nb_S,   &h->nb_count};
-        for (DevBuf *b : bufs) b->release();
+        for (DevBuf *b : bufs) b->release(h->stream);
         device_wide([&] {
             if (h->status_host) (void)hipHostFree(h->status_host);
             if (h->staging) (void)hipHostFree(h->staging);

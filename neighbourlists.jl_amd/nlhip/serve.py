@@ -116,14 +116,22 @@ class _Resident:
             self.handle = Handle()
         self.lock = threading.Lock()
         self.built = False
+        self.rebuilds = 0
+
+    # rebuilds before the pipeline is captured as a graph: capturing and
+    # instantiating costs milliseconds, more than a few eager rebuilds
+    GRAPH_AFTER = 3
 
     def build(self, X: np.ndarray) -> int:
         """First build, or a rebuild on new positions (upload into the
-        resident buffer, replay of the captured pipeline)."""
+        resident buffer; eager for the first few rebuilds, then replays of
+        the captured pipeline)."""
         n = C.c_int64()
         X = np.ascontiguousarray(X, dtype=np.float64)
         if self.built:
-            check(lib().nlh_batch_rebuild(self.handle.ptr, X.ctypes.data, 0, C.byref(n)), "rebuild")
+            flags = _lib.NLH_NO_GRAPH if self.rebuilds < self.GRAPH_AFTER else 0
+            self.rebuilds += 1
+            check(lib().nlh_batch_rebuild(self.handle.ptr, X.ctypes.data, flags, C.byref(n)), "rebuild")
         else:
             check(lib().nlh_neighbour_list(self.handle.ptr, X.ctypes.data, self.nat, C.byref(self.geom), 0,
                                            C.byref(n)), "neighbour_list")
