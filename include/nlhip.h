@@ -164,14 +164,14 @@ int nlh_batch_pair_offsets(nlh_handle *h, int64_t *pair_offsets /* nsys + 1, hos
  * neighbour_list (src/cell_list.jl:897-916). */
 int nlh_batch_rebuild(nlh_handle *h, const double *X_batch, uint32_t flags, int64_t *total_pairs);
 
-/* Launch configuration of the build.  brick_atoms: expected atoms (halo
- * included) of one count-pass brick, which sets the brick shape of each
- * system (64..2048; the LDS holds 2048).  fill_block: threads per fill
+/* Launch configuration of the build.  brick_atoms: largest expected atom
+ * count (halo included) of one count-pass brick, which sets the brick shape
+ * of each system (64..2048; default 920; at most 90% of the 4 x count_block
+ * atoms a brick's LDS holds is used).  fill_block: threads per fill
  * workgroup, a multiple of 64 up to 1024.  fill_pairs: output pairs per
  * fill wave (128, 256 or 512; each wave writes one aligned tile of the
  * pair arrays; default 256).  count_block: threads per count-pass brick workgroup, 256 or
- * 512, or 0 for the default (256); a 256-thread brick holds at most 768
- * expected atoms (default brick_atoms 768).  cell_sort: 0 (default) sorts
+ * 512, or 0 for the default (256).  cell_sort: 0 (default) sorts
  * the atoms by cell through buckets of cells counted in LDS wherever the
  * cell grid allows (up to 8192 buckets of 4096 cells), 1 uses one global
  * atomic histogram over the cells.
@@ -188,7 +188,7 @@ int nlh_set_tuning(nlh_handle *h, const nlh_tuning *t);
 int nlh_get_tuning(nlh_handle *h, nlh_tuning *t);
 /* Time the current batch (set up by a previous build on this handle) under
  * each candidate configuration (count_block in {256, 512} x brick_atoms
- * in {384, 768, 1536} (768 at most for 256) x fill_block in {128, 256, 512}
+ * in {460, 920, 1840} (920 at most for 256) x fill_block in {128, 256, 512}
  * x fill_pairs in {256, 512}, reps timed rebuilds each, HIP events on the
  * handle's stream), keep the fastest and return it in *best.  Leaves the
  * handle holding a build of X_batch under the winning configuration. */

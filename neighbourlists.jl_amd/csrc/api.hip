@@ -170,7 +170,8 @@ struct nlh_handle {
     nlh::PipelineBufs last_bufs;         // ... on these buffers and sizes
     int64_t graph_launches = 0;
     // launch configuration (nlh_set_tuning / nlh_autotune)
-    nlh_tuning tune = {768, 256, 256, 0, 0};
+    nlh_tuning tune = {920, 256, 256, 0, 0};
+    int cus = 256;   // compute units of the device
 
     // count-pass workgroup: 4 waves per brick unless tuned (four 36 KB
     // bricks share a CU; 8-wave bricks of 72 KB fit two)
@@ -398,8 +399,8 @@ void drop_graph(nlh_handle *h)
 }
 
 // Pair-search brick of a system: the largest of a fixed ladder of shapes
-// whose halo is expected to fit the LDS budget at this system's mean atom
-// density per cell (clipped to the cell grid).
+// whose halo atoms, expected at this system's mean atom density per cell,
+// stay within the budget (clipped to the cell grid).
 void choose_brick(const nlh_geometry &g, int64_t nat, int32_t budget, int32_t halo_max, int32_t out[3],
                   int64_t min_bricks = 0)
 {
@@ -415,7 +416,7 @@ void choose_brick(const nlh_geometry &g, int64_t nat, int32_t budget, int32_t ha
             halo *= (double)(d[k] + 2);
             nb *= (g.ncells[k] + d[k] - 1) / d[k];
         }
-        if (halo * per_cell * 1.25 <= (double)budget && halo <= (double)halo_max) {
+        if (halo * per_cell <= (double)budget && halo <= (double)halo_max) {
             // the first shape that fits; a smaller one while the grid is too
             // coarse to fill the device, down to one home cell per wave
             std::memcpy(out, d, sizeof d);
@@ -426,7 +427,10 @@ void choose_brick(const nlh_geometry &g, int64_t nat, int32_t budget, int32_t ha
     if (!found) out[0] = out[1] = out[2] = 1;
 }
 
-constexpr int64_t kMinBricks = 1024;
+// A brick's LDS holds 4 BT halo atoms; the expected halo may fill 90% of it
+// (the rest absorbs the density's fluctuation: a brick that overflows is
+// counted from global memory instead).
+constexpr double kHaloFill = 0.9;
 
 // Pair-search bricks of every system for the current brick budget: brick
 // shape per system and the first brick of each system (host tables; the
@@ -439,11 +443,13 @@ void assign_bricks(nlh_handle *h)
     for (int s = 0; s < nsys; ++s) {
         nlh::SysDev &sd = h->sys_host[s];
         // a single system splits into at least two rounds of bricks over the
-        // device (2 bricks per CU fit its LDS)
+        // device: four 256-thread bricks (36 KB of LDS each) or two
+        // 512-thread bricks fit a CU
         // the LDS of a BT-thread brick holds 4 BT halo atoms and BT halo cells
         const int bt = h->count_block();
-        choose_brick(h->geoms[s], sd.nat, std::min(h->tune.brick_atoms, 3 * bt), bt, sd.brick,
-                     nsys == 1 ? kMinBricks : 0);
+        const int64_t min_bricks = nsys == 1 ? 2 * (int64_t)h->cus * (bt == 256 ? 4 : 2) : 0;
+        choose_brick(h->geoms[s], sd.nat, std::min(h->tune.brick_atoms, (int32_t)(kHaloFill * 4 * bt)), bt,
+                     sd.brick, min_bricks);
         sd.brick_base = (int32_t)bricks;
         h->brick_base_host[s] = (int32_t)bricks;
         bricks += (int64_t)((sd.n[0] + sd.brick[0] - 1) / sd.brick[0]) * ((sd.n[1] + sd.brick[1] - 1) / sd.brick[1]) *
@@ -1012,6 +1018,7 @@ int nlh_create(nlh_handle **out)
         try {
             HIPC(hipSetDevice(h->device));
             keep_pool_blocks(h->device);
+            HIPC(hipDeviceGetAttribute(&h->cus, hipDeviceAttributeMultiprocessorCount, h->device));
             HIPD(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
             HIPD(hipHostMalloc((void **)&h->status_host, sizeof(nlh::Status),
                                hipHostMallocMapped | hipHostMallocCoherent));
@@ -1372,7 +1379,7 @@ int nlh_autotune(nlh_handle *h, const double *X_batch, uint32_t flags, int32_t r
         float win_ms = 0.0f;
         bool have = false;
         try {
-            static const int32_t bricks[] = {384, 768, 1536};
+            static const int32_t bricks[] = {460, 920, 1840};
             static const int32_t fills[] = {128, 256, 512};
             static const int32_t tiles[] = {256, 512};
             static const int32_t blocks[] = {256, 512};
@@ -1380,7 +1387,7 @@ int nlh_autotune(nlh_handle *h, const double *X_batch, uint32_t flags, int32_t r
             for (int32_t ba : bricks)
                 for (int32_t fb : fills)
                     for (int32_t fa : tiles) {
-                    if (cb == 256 && ba > 768) continue;   // above what a 256-thread brick holds
+                    if (cb == 256 && ba > 920) continue;   // above what a 256-thread brick holds
                     apply_tuning(h, nlh_tuning{ba, fb, fa, cb, start.cell_sort});
                     rebuild_eager(h, flags & ~NLH_NO_SYNC);   // warm-up (and buffer growth)
                     float tot = 0.0f;

@@ -133,7 +133,7 @@ def test_autotune_keeps_results():
     bb = BatchBuilder()
     before = bb.run(systems, want_R=True)
     best = bb.autotune(X0, reps=1)
-    assert best.brick_atoms in (384, 768, 1536) and best.fill_block in (128, 256, 512)
+    assert best.brick_atoms in (460, 920, 1840) and best.fill_block in (128, 256, 512)
     assert best.count_block in (256, 512)
     assert best.fill_pairs in (256, 512)
     t = bb.handle.get_tuning()
