@@ -313,7 +313,10 @@ __global__ __launch_bounds__(kScanThreads) void scan_pairs_kernel(
         const int32_t o = __shfl_xor(mx, off, kWave);
         mx = o > mx ? o : mx;
     }
-    if (lane_id() == 0 && mx > 0) atomicMax(&status->max_count, mx);
+    // one atomic per block, and only while it can raise the maximum: atomics
+    // on the one status word serialise (one per wave cost 20 us at 1M atoms)
+    __shared__ int32_t wmax[kScanThreads / kWave];
+    if (lane_id() == 0) wmax[threadIdx.x / kWave] = mx;
     __shared__ int64_t ex_sh[kTileSlots];
     __shared__ int32_t cnt_sh[kTileSlots];
     int64_t run = boff + ex;
@@ -325,6 +328,13 @@ __global__ __launch_bounds__(kScanThreads) void scan_pairs_kernel(
         run += v[q];
     }
     __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t bm = 0;
+#pragma unroll
+        for (int w = 0; w < kScanThreads / kWave; ++w) bm = wmax[w] > bm ? wmax[w] : bm;
+        if (bm > __hip_atomic_load(&status->max_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&status->max_count, bm);
+    }
     const int64_t tsz = (int64_t)1 << tile_shift;
     for (int q = 0; q < kScanItems; ++q) {
         const int i = q * kScanThreads + (int)threadIdx.x;
