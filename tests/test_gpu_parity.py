@@ -306,26 +306,52 @@ def test_fill_tiles_over_many_empty_atoms(fill_pairs):
     _same(nl, ref)
 
 
+def _cell_lists_both_sorts(X, rc, C, pbc, ref):
+    from nlhip import _lib
+
+    for cs in (0, 1):
+        h = _lib.Handle()
+        h.set_tuning(920, 256, 256, 0, cs)
+        cl = nlhip.build_cell_list(X, rc, C, pbc, handle=h)
+        for k in ("perm", "cell_id", "cell_offsets"):
+            assert np.array_equal(getattr(cl, k), getattr(ref, k)), (k, cs)
+        h.close()
+
+
 def test_scan_paths():
-    """Both scan forms: a single pass up to 64 tiles of 2048 items, reduce ->
-    tile sums -> downsweep beyond.  300k atoms: 147 tiles of pair counts
-    (three kernels), 23 of cells (single pass); a sparse 100k-atom box:
-    over 3 million cells (three kernels), 49 tiles of pair counts (single
-    pass)."""
+    """Both scan forms: a single pass up to 64 tiles of 2048 items (128 for
+    the bucketed sort's counts), reduce -> tile sums -> downsweep beyond.
+    300k atoms: 147 tiles of pair counts (three kernels), 23 of cells for the
+    atomic sort (single pass); a sparse 100k-atom box: over 3 million cells
+    (the atomic sort's three-kernel cell scan; 781 buckets of 4096 cells for
+    the bucketed one), 49 tiles of pair counts (single pass)."""
     s = workloads.random_cubic(300_000, density=0.05, seed=95)
     print("scan paths: 300k cube", flush=True)
     nl = nlhip.neighbour_list(s.X, 5.0, s.cell, s.pbc)
     ref = oracle.neighbour_list(s.X, 5.0, s.cell, s.pbc)
     _same(nl, ref, R=False)
-    cl = nlhip.build_cell_list(s.X, 5.0, s.cell, s.pbc)
-    for k in ("perm", "cell_id", "cell_offsets"):
-        assert np.array_equal(getattr(cl, k), getattr(ref, k)), k
+    _cell_lists_both_sorts(s.X, 5.0, s.cell, s.pbc, ref)
     s = workloads.random_cubic(100_000, density=0.002, seed=96, cutoff=2.5)
     print("scan paths: sparse 100k, 3.2M cells", flush=True)
     _, ref = _check(s.X, 2.5, s.cell, (True, False, True))
-    cl = nlhip.build_cell_list(s.X, 2.5, s.cell, (True, False, True))
+    _cell_lists_both_sorts(s.X, 2.5, s.cell, (True, False, True), ref)
+
+
+def test_huge_grid_takes_the_atomic_sort():
+    """A grid beyond the bucketed sort's reach (8192 buckets of 4096 cells):
+    500 atoms (200 of them in one cluster) in a 2000 A box at cutoff 5 A
+    have 400^3 = 64M cells, and the default sort falls back to the atomic
+    histogram."""
+    rng = np.random.default_rng(97)
+    L = 2000.0
+    X = np.concatenate([rng.random((300, 3)) * L, rng.random((200, 3)) * 20.0 + 100.0])   # a cluster with pairs
+    C = np.eye(3) * L
+    print("huge grid: 64M cells", flush=True)
+    cl = nlhip.build_cell_list(X, 5.0, C, (True, True, True))
+    ref = oracle.neighbour_list(X, 5.0, C, (True, True, True))
     for k in ("perm", "cell_id", "cell_offsets"):
         assert np.array_equal(getattr(cl, k), getattr(ref, k)), k
+    _same(nlhip.materialize_pairlist(cl), ref, R=False)
 
 
 def test_repeated_builds_replay_the_pipeline():
