@@ -234,6 +234,21 @@ class Server:
         bb = self._builders[local["slot"]]
         if bb is None:
             bb = self._builders[local["slot"]] = BatchBuilder(self.int_type)
+        if len(reqs) == 1:
+            # a batch of one: the single-system call on the slot's handle
+            # (no packing, system table or per-system offsets to fetch)
+            r = reqs[0]
+            g = geometry(r.cell, r.cutoff, r.pbc, _lib.ti_bytes_of(self.int_type))
+            X = np.ascontiguousarray(r.X, dtype=np.float64)
+            n = C.c_int64()
+            check(lib().nlh_neighbour_list(bb.handle.ptr, X.ctypes.data, X.shape[0], C.byref(g), 0, C.byref(n)),
+                  "neighbour_list")
+            out = None
+            if self.keep:
+                from .api import _collect_pairs
+                out = _collect_pairs(bb.handle, g, X, int(n.value), False, None)
+            end = time.perf_counter() - self.t0
+            return [("ok", Completion(r.rid, r.system_id, int(n.value), r.arrival, start, end, 1, out), nbytes)]
         systems = [System(r.X, r.cell, r.pbc, r.cutoff) for r in reqs]
         bb.run(systems, copy=False)
         po = np.zeros(len(reqs) + 1, dtype=np.int64)
@@ -397,8 +412,10 @@ class Server:
             batch, atoms, nb = [], 0, 0
             rest = []
             for q in waiting[idx:]:
+                # (the first request of a batch goes in even above batch_atoms)
                 if (q.nat < self.small_atoms and q.system_id not in inflight_systems and
-                        atoms + q.nat <= self.batch_atoms and q.system_id not in {b.system_id for b in batch}):
+                        (not batch or atoms + q.nat <= self.batch_atoms) and
+                        q.system_id not in {b.system_id for b in batch}):
                     e = estimate_bytes(q.nat, q.cell, q.cutoff)
                     if not self._fits(nb + e):
                         rest.append(q)

@@ -49,6 +49,20 @@ def test_alone_vs_mixed(max_residents):
             assert np.array_equal(m.result.j, ref.j) and np.array_equal(m.result.S, ref.S)
 
 
+def test_single_request_units_match_alone():
+    # batch_atoms=1: every small request is a unit of its own (the
+    # single-system call on the worker's handle), large ones are residents
+    reqs = _stream(24, seed=25, nmax=8000)
+    srv = serve.Server(workers=2, small_atoms=4096, batch_atoms=1, keep_results=True)
+    mixed = srv.serve(reqs, realtime=False)
+    srv.close()
+    assert max(c.batch for c in mixed) == 1
+    alone = serve.serve_plain(reqs, realtime=False, keep_results=True)
+    for m, a in zip(mixed, alone):
+        assert m.npairs == a.npairs
+        _same(m.result, a.result)
+
+
 def test_open_loop_latency_report():
     reqs = workloads.served_stream(60, rate=2000.0, seed=22, nmax=20000)
     srv = serve.Server(workers=2, small_atoms=4096)
