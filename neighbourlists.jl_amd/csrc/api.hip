@@ -74,6 +74,7 @@ int guarded(F &&f)
 struct DevBuf {
     void *p = nullptr;
     size_t cap = 0;
+    int grows = 0;   // reallocations after the first allocation
     template <typename T>
     T *as() const { return static_cast<T *>(p); }
     // returns true when (re)allocated
@@ -88,8 +89,11 @@ struct DevBuf {
     {
         if (bytes == 0) bytes = 16;
         if (bytes <= cap) return false;
-        // a quarter of headroom: builds of varying size regrow rarely
-        const size_t nb = std::max(bytes + bytes / 4, (size_t)256);
+        // a quarter of headroom at first; a buffer that has to grow again
+        // serves builds of varying size (a serving worker, a recycled
+        // resident), and doubles from then on
+        if (p) ++grows;
+        const size_t nb = std::max(grows ? 2 * bytes : bytes + bytes / 4, (size_t)256);
         device_wide([&] {
             if (p) HIPD(hipFreeAsync(p, st));
             p = nullptr;
@@ -109,6 +113,7 @@ struct DevBuf {
         });
         p = nullptr;
         cap = 0;
+        grows = 0;
     }
 };
 

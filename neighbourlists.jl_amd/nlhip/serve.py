@@ -32,6 +32,7 @@ the alone-vs-mixed test checks.
 from __future__ import annotations
 
 import ctypes as C
+import gc
 import queue
 import threading
 import time
@@ -333,6 +334,11 @@ class Server:
             if estimate_bytes(r.nat, r.cell, r.cutoff) > self.budget:
                 raise _lib.NLHMemoryError(_lib.NLH_ENOMEM, f"request {r.rid} ({r.nat} atoms) alone exceeds the "
                                                            f"HBM budget of {self.budget} bytes")
+        # no cyclic garbage collection while serving: a collection pass over a
+        # large process stalls every thread for milliseconds (reference
+        # counting still frees the per-request objects)
+        gc_on = gc.isenabled()
+        gc.disable()
         # a fresh count of free workers for this call's threads (the previous
         # call's threads have exited)
         self._free_workers = threading.Semaphore(0)
@@ -390,6 +396,8 @@ class Server:
                 self._work.put(None)
             for t in self._threads:
                 t.join(timeout=60)
+            if gc_on:
+                gc.enable()
         return sorted(out, key=lambda c: c.rid)
 
     def _next_item(self, waiting: list[Request], inflight_systems: set):
@@ -458,8 +466,21 @@ def serve_plain(requests: list[Request], *, realtime: bool = True, int_type=Int3
     ti = _lib.ti_bytes_of(int_type)
     order = sorted(requests, key=lambda r: r.arrival)
     out = []
+    gc_on = gc.isenabled()
+    gc.disable()   # as in Server.serve
     t0 = time.perf_counter()
     n = C.c_int64()
+    try:
+        _plain_loop(h, ti, order, out, t0, n, realtime, keep_results)
+    finally:
+        if gc_on:
+            gc.enable()
+    if handle is None:
+        h.close()
+    return sorted(out, key=lambda c: c.rid)
+
+
+def _plain_loop(h, ti, order, out, t0, n, realtime, keep_results):
     for r in order:
         if realtime:
             while (wait := r.arrival - (time.perf_counter() - t0)) > 0:
@@ -474,9 +495,6 @@ def serve_plain(requests: list[Request], *, realtime: bool = True, int_type=Int3
             res = _collect_pairs(h, g, X, int(n.value), False, None)
         end = time.perf_counter() - t0
         out.append(Completion(r.rid, r.system_id, int(n.value), r.arrival, start, end, 1, res))
-    if handle is None:
-        h.close()
-    return sorted(out, key=lambda c: c.rid)
 
 
 def summarize(done: list[Completion], wall: float | None = None) -> dict:
