@@ -308,7 +308,7 @@ def main():
                     help="run one workload alone (profiler runs); prints its own JSON")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU; multi-GPU runs are not part of this build", file=sys.stderr)
+        print("bench.py measures one GPU only (--gpus 1): the library builds each neighbour list on one device", file=sys.stderr)
         sys.exit(2)
 
     import nlhip

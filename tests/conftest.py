@@ -8,4 +8,4 @@ for p in (os.path.join(REPO, "neighbourlists.jl_amd"), REPO):
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device); run on the GPU box")
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
