@@ -219,13 +219,19 @@ def pbc_bytes(pbc) -> np.ndarray:
     return p.astype(np.uint8)
 
 
+_Cell9 = C.c_double * 9
+_Pbc3 = C.c_uint8 * 3
+
+
 def geometry(cell, cutoff: float, pbc, ti_bytes: int = 4) -> Geometry:
-    """nlh_geometry_init: analyze_cell + overflow check + nxyz."""
+    """nlh_geometry_init: analyze_cell + overflow check + nxyz.  (The arrays
+    go over as ctypes copies of the numpy buffers: `ndarray.ctypes.data_as`
+    costs several times the call itself, and this runs once per request.)"""
     g = Geometry()
     c = cell_colmajor(cell)
     p = pbc_bytes(pbc)
-    check(lib().nlh_geometry_init(c.ctypes.data_as(C.POINTER(C.c_double)), float(cutoff),
-                                  p.ctypes.data_as(C.POINTER(C.c_uint8)), int(ti_bytes), C.byref(g)),
+    check(lib().nlh_geometry_init(_Cell9.from_buffer_copy(c), float(cutoff), _Pbc3.from_buffer_copy(p),
+                                  int(ti_bytes), C.byref(g)),
           "geometry")
     return g
 

@@ -33,7 +33,9 @@ from __future__ import annotations
 
 import ctypes as C
 import gc
+import math
 import queue
+import sys
 import threading
 import time
 from collections import OrderedDict
@@ -80,16 +82,24 @@ class Completion:
         return self.end - self.arrival
 
 
+def _cross(u, v):
+    return (u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0])
+
+
 def estimate_bytes(nat: int, cell, cutoff: float, ti_bytes: int = 4) -> int:
     """Device memory of one build: positions, cell list, neighbour slots and
-    pair outputs (pair count from the mean density; over-estimates by ~25%)."""
-    cell = np.asarray(cell, dtype=np.float64)
-    vol = abs(float(np.linalg.det(cell)))
+    pair outputs (pair count from the mean density; over-estimates by ~25%).
+    Scalar arithmetic on the 3x3 cell: numpy's per-call cost on 3-vectors
+    would put ~0.1 ms on the scheduler for every admitted request."""
+    a = np.asarray(cell, dtype=np.float64).tolist()
+    faces = [_cross(a[(k + 1) % 3], a[(k + 2) % 3]) for k in range(3)]   # |face k| x height k = volume
+    vol = abs(a[0][0] * faces[0][0] + a[0][1] * faces[0][1] + a[0][2] * faces[0][2])
     rho = nat / vol if vol > 0 else 0.0
     nb = rho * 4.18879020478639 * cutoff ** 3
-    heights = np.array([vol / max(np.linalg.norm(np.cross(cell[(k + 1) % 3], cell[(k + 2) % 3])), 1e-300)
-                        for k in range(3)])
-    M = float(np.prod(np.maximum(np.floor(heights / cutoff), 1.0)))
+    M = 1.0
+    for f in faces:
+        h = vol / max(math.sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]), 1e-300)
+        M *= max(math.floor(h / cutoff), 1.0)
     cap = int((nb * 1.5 + 16 + 7) // 8 * 8)
     pairs = nat * (nb * 1.25 + 4)
     per_atom = 24 + 6 * 4 + 32 + 12 + 2 * ti_bytes + 8 + ti_bytes + 8 * cap
@@ -197,6 +207,7 @@ class Server:
         # one batch builder per worker slot, kept across serve() calls: its
         # device buffers only grow, so a warm server allocates nothing
         self._builders: list[BatchBuilder | None] = [None] * self.nworkers
+        self._est: dict[int, int] = {}   # id(request) -> estimate_bytes, for one serve() call
         self.t0 = 0.0
 
     # ------------------------------------------------------------------ workers
@@ -334,11 +345,7 @@ class Server:
             if estimate_bytes(r.nat, r.cell, r.cutoff) > self.budget:
                 raise _lib.NLHMemoryError(_lib.NLH_ENOMEM, f"request {r.rid} ({r.nat} atoms) alone exceeds the "
                                                            f"HBM budget of {self.budget} bytes")
-        # no cyclic garbage collection while serving: a collection pass over a
-        # large process stalls every thread for milliseconds (reference
-        # counting still frees the per-request objects)
-        gc_on = gc.isenabled()
-        gc.disable()
+        self._est = {}
         # a fresh count of free workers for this call's threads (the previous
         # call's threads have exited)
         self._free_workers = threading.Semaphore(0)
@@ -361,6 +368,16 @@ class Server:
             inflight_systems.discard(c.system_id)
             out.append(c)
 
+        # no cyclic garbage collection while serving: a collection pass over a
+        # large process stalls every thread for milliseconds (reference
+        # counting still frees the per-request objects)
+        gc_on = gc.isenabled()
+        gc.disable()
+        # the scheduler and the workers hand the interpreter to each other
+        # within 0.2 ms rather than the default 5 ms: a thread woken by a
+        # completion or an arrival otherwise waits out a busy thread's slice
+        switch = sys.getswitchinterval()
+        sys.setswitchinterval(min(switch, 2e-4))
         try:
             while len(out) < n:
                 now = time.perf_counter() - self.t0
@@ -396,9 +413,18 @@ class Server:
                 self._work.put(None)
             for t in self._threads:
                 t.join(timeout=60)
+            sys.setswitchinterval(switch)
             if gc_on:
                 gc.enable()
         return sorted(out, key=lambda c: c.rid)
+
+    def _estimate(self, r: Request) -> int:
+        """estimate_bytes of a request, computed once (a request may be
+        looked at by several dispatch passes before it is admitted)."""
+        e = self._est.get(id(r))   # the stream's requests stay alive until serve() returns
+        if e is None:
+            e = self._est[id(r)] = estimate_bytes(r.nat, r.cell, r.cutoff)
+        return e
 
     def _next_item(self, waiting: list[Request], inflight_systems: set):
         """The next unit of work from the admitted head of the queue: one
@@ -408,7 +434,7 @@ class Server:
                 continue                     # a system is built by one worker at a time
             if r.nat >= self.small_atoms:
                 res = self.residents.get(r.system_id)
-                nb = 0 if res is not None else estimate_bytes(r.nat, r.cell, r.cutoff)
+                nb = 0 if res is not None else self._estimate(r)
                 if not self._fits(nb):
                     return None
                 with self._lock:
@@ -424,7 +450,7 @@ class Server:
                 if (q.nat < self.small_atoms and q.system_id not in inflight_systems and
                         (not batch or atoms + q.nat <= self.batch_atoms) and
                         q.system_id not in {b.system_id for b in batch}):
-                    e = estimate_bytes(q.nat, q.cell, q.cutoff)
+                    e = self._estimate(q)
                     if not self._fits(nb + e):
                         rest.append(q)
                         continue
