@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -134,6 +135,7 @@ struct nlh_handle {
     nlh::Status *status_host = nullptr;   // pinned, mapped: the fill kernel writes it
     nlh::Status *status_host_dev = nullptr;   // its device address
     bool status_pub_pending = false;      // the last launched fill publishes the status
+    bool inflight = false;                // a replay returned while its fill still writes pairs
     bool graph_publishes = false;         // the captured pipeline ends with a fill
     void *staging = nullptr;              // pinned upload staging for system tables
     size_t staging_cap = 0;
@@ -393,8 +395,11 @@ void ensure_staging(nlh_handle *h, size_t bytes)
     h->staging_cap = nb;
 }
 
+void settle(nlh_handle *h);
+
 void drop_graph(nlh_handle *h)
 {
+    settle(h);   // an executable graph is destroyed only once its replay is done
     device_wide([&] {
         if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
         if (h->graph) (void)hipGraphDestroy(h->graph);
@@ -712,10 +717,53 @@ void finish_status(nlh_handle *h)
         HIPC(hipMemcpyAsync(h->status_host, h->status.p, sizeof(nlh::Status), hipMemcpyDeviceToHost, h->stream));
     HIPC(hipStreamSynchronize(h->stream));
     h->status_pub_pending = false;
+    h->inflight = false;
     if (h->status_host->scan_error) {
         h->built = false;
         throw Error(NLH_EDEVICE, "internal: a scan block waited too long for its predecessors");
     }
+}
+
+// The status of a replayed pipeline, read as soon as its fill has published
+// it: the fill does so as it starts, so the host goes on (checks, the
+// caller's next request, the next launch) while the fill writes the pairs,
+// and the pipeline's last stage overlaps the host's turnaround.  The handle
+// is then `inflight`: settle() waits for the fill before anything hands out
+// or replaces what it writes.  Stops early when the stream has drained (the
+// flag must then be set) or reports an error.
+void await_published(nlh_handle *h)
+{
+    const int32_t *pub = &h->status_host->published;
+    for (unsigned spin = 1;; ++spin) {
+        if (__atomic_load_n(pub, __ATOMIC_ACQUIRE)) {
+            h->inflight = true;
+            break;
+        }
+        if ((spin & 255) == 0) {
+            const hipError_t q = hipStreamQuery(h->stream);
+            if (q == hipSuccess) {
+                if (!__atomic_load_n(pub, __ATOMIC_ACQUIRE))
+                    throw Error(NLH_EDEVICE, "internal: the pipeline ended without publishing its status");
+                break;
+            }
+            if (q != hipErrorNotReady) HIPC(q);
+        }
+        __builtin_ia32_pause();
+    }
+    h->status_pub_pending = false;
+    if (h->status_host->scan_error) {
+        h->built = false;
+        settle(h);
+        throw Error(NLH_EDEVICE, "internal: a scan block waited too long for its predecessors");
+    }
+}
+
+// Wait for a fill that was still writing when its call returned.
+void settle(nlh_handle *h)
+{
+    if (!h->inflight) return;
+    HIPC(hipStreamSynchronize(h->stream));
+    h->inflight = false;
 }
 
 void check_positions(nlh_handle *h)
@@ -870,6 +918,7 @@ bool rebuild_graph(nlh_handle *h, bool want_R)
     const nlh::PipelineBufs b = h->bufs();
     if (!h->graph_exec || h->graph_want_R != want_R || std::memcmp(&b, &h->graph_bufs, sizeof b) != 0)
         capture_graph(h, want_R);
+    if (h->graph_publishes) h->status_host->published = 0;   // the previous fill has published (or ended)
     HIPC(hipGraphLaunch(h->graph_exec, h->stream));
     h->pair_lookback_clean = false;   // the replay's pair scan used them
     h->status_pub_pending = h->graph_publishes;
@@ -877,10 +926,14 @@ bool rebuild_graph(nlh_handle *h, bool want_R)
     h->built = true;
     h->materialized = false;
     h->ev_build = h->ev_pairs = false;
-    finish_status(h);
+    if (h->graph_publishes)
+        await_published(h);
+    else
+        finish_status(h);
     check_positions(h);
     const nlh::Status &st = *h->status_host;
     if (st.npairs > h->pair_capacity || st.max_count > h->slot_cap || (st.wide && !h->wide)) return false;
+    if (h->nsys > 1) settle(h);   // a batch copies its per-system pair offsets back (a plain copy)
     finish_pairs(h, want_R);
     return true;
 }
@@ -1097,6 +1150,7 @@ int nlh_build_cell_list(nlh_handle *h, const double *X, int64_t nat, const nlh_g
         if (!h || !geom) throw Error(NLH_EINVAL, "NULL argument");
         if (nat < 0) throw Error(NLH_EINVAL, "nat < 0");
         HIPC(hipSetDevice(h->device));
+        settle(h);
         h->last_single = false;   // not a single full build: no repeat to replay
         h->built = false;
         drop_graph(h);
@@ -1117,6 +1171,7 @@ int nlh_materialize(nlh_handle *h, uint32_t flags, int64_t *npairs)
     return guarded([&] {
         if (!h) throw Error(NLH_EINVAL, "handle is NULL");
         HIPC(hipSetDevice(h->device));
+        settle(h);
         materialize(h, flags);
         if (npairs) *npairs = h->npairs;
     });
@@ -1157,6 +1212,8 @@ int nlh_pairlist_device(nlh_handle *h, nlh_pairlist_view *out)
     return guarded([&] {
         if (!h || !out) throw Error(NLH_EINVAL, "NULL argument");
         if (!h->materialized) throw Error(NLH_ESTATE, "no materialised pair list");
+        HIPC(hipSetDevice(h->device));
+        settle(h);   // every output of the last build is complete
         out->nat = h->nat;
         out->npairs = h->npairs;
         out->i = h->i.p;
@@ -1174,6 +1231,8 @@ int nlh_celllist_device(nlh_handle *h, nlh_celllist_view *out)
         if (!h || !out) throw Error(NLH_EINVAL, "NULL argument");
         if (!h->built) throw Error(NLH_ESTATE, "no cell list built");
         if (h->nsys != 1) throw Error(NLH_ESTATE, "cell-list views are per single-system build");
+        HIPC(hipSetDevice(h->device));
+        settle(h);   // every output of the last build is complete
         out->nat = h->nat;
         out->ncells_total = h->M;
         out->X_orig = h->X_dev;
@@ -1190,6 +1249,7 @@ int nlh_copy_pairlist(nlh_handle *h, void *i, void *j, void *S, void *first, dou
         if (!h) throw Error(NLH_EINVAL, "handle is NULL");
         if (!h->materialized) throw Error(NLH_ESTATE, "no materialised pair list");
         HIPC(hipSetDevice(h->device));
+        settle(h);
         HIPC(hipStreamSynchronize(h->stream));
         const size_t P = (size_t)h->npairs, ti = (size_t)h->ti_bytes;
         if (P > 0) {
@@ -1211,6 +1271,7 @@ int nlh_copy_celllist(nlh_handle *h, double *X_sorted, void *perm, void *cell_id
         if (!h) throw Error(NLH_EINVAL, "handle is NULL");
         if (!h->built) throw Error(NLH_ESTATE, "no cell list built");
         HIPC(hipSetDevice(h->device));
+        settle(h);
         const size_t n = (size_t)h->nat, ti = (size_t)h->ti_bytes;
         if (X_sorted && n > 0) {
             h->Xsorted.ensure(h->stream, n * 24);
@@ -1236,6 +1297,7 @@ int nlh_neighbours(nlh_handle *h, int32_t sys, int64_t i, int64_t cap, void *j, 
         if (i < 1 || i > sd.nat) throw Error(NLH_EINVAL, "atom index out of range (1-based)");
         if (cap < 0) throw Error(NLH_EINVAL, "cap < 0");
         HIPC(hipSetDevice(h->device));
+        settle(h);
         const int ti = h->ti_bytes;
         const size_t c1 = (size_t)std::max<int64_t>(cap, 1);
         h->nb_j.ensure(h->stream, c1 * ti);
@@ -1279,6 +1341,7 @@ int nlh_set_profiling(nlh_handle *h, int enable)
     return guarded([&] {
         if (!h) throw Error(NLH_EINVAL, "handle is NULL");
         HIPC(hipSetDevice(h->device));
+        settle(h);
         if (enable && !h->ev[0])
             for (hipEvent_t &e : h->ev) HIPD(hipEventCreate(&e));
         h->profile = enable != 0;
@@ -1315,6 +1378,7 @@ int nlh_batch_neighbour_list(nlh_handle *h, int32_t nsys, const double *X_batch,
     return guarded([&] {
         if (!h || !atom_offsets || !geoms) throw Error(NLH_EINVAL, "NULL argument");
         HIPC(hipSetDevice(h->device));
+        settle(h);
         h->last_single = false;   // not a single full build: no repeat to replay
         h->built = false;
         drop_graph(h);
@@ -1356,6 +1420,7 @@ int nlh_set_tuning(nlh_handle *h, const nlh_tuning *t)
     return guarded([&] {
         if (!h || !t) throw Error(NLH_EINVAL, "NULL argument");
         HIPC(hipSetDevice(h->device));
+        settle(h);
         apply_tuning(h, *t);
     });
 }
@@ -1375,6 +1440,7 @@ int nlh_autotune(nlh_handle *h, const double *X_batch, uint32_t flags, int32_t r
         if (h->sys_host.empty()) throw Error(NLH_ESTATE, "autotune needs a previous build on this handle");
         if (reps < 1 || reps > 100) throw Error(NLH_EINVAL, "reps must be in [1, 100]");
         HIPC(hipSetDevice(h->device));
+        settle(h);
         upload_positions(h, X_batch, flags);
         hipEvent_t e0 = nullptr, e1 = nullptr;
         HIPD(hipEventCreate(&e0));

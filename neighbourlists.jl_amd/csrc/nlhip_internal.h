@@ -140,7 +140,7 @@ __host__ __device__ inline void unpack_wind(uint32_t w, int32_t &wx, int32_t &wy
 struct Status {
     int64_t npairs;          // total pairs over the batch
     int32_t scan_error;      // a scan block's look-back exceeded its bound (internal error)
-    int32_t spare;
+    int32_t published;       // host copy only: set (last) by the fill that published the rest
     int32_t overflow;        // pair capacity exceeded: outputs not written
     int32_t bad_position;    // non-finite / out-of-range position seen
     int32_t wide;            // a cell shift does not fit a packed slot entry

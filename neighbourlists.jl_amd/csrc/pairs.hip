@@ -1001,12 +1001,22 @@ __global__ __launch_bounds__(1024) void fill_kernel(
     // the output (an XCD-contiguous block order measured much slower)
     const int64_t w = (int64_t)blockIdx.x * (blockDim.x / kWave) + sgpr((int)(threadIdx.x / kWave));   // wave-uniform
     const int64_t Q0 = w * TP;
-    if (blockIdx.x == 0 && threadIdx.x < sizeof(Status) / 4) {
+    if (blockIdx.x == 0 && threadIdx.x < kWave) {
         // the status is final once the fill runs: publish it to the host's
-        // pinned copy (system-scope stores), so the host only has to wait
-        const int32_t v = reinterpret_cast<const int32_t *>(status)[threadIdx.x];
-        __hip_atomic_store(reinterpret_cast<int32_t *>(status_pub) + threadIdx.x, v, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_SYSTEM);
+        // pinned copy (system-scope stores), then mark it published, so the
+        // host can go on while the fill writes the pairs
+        constexpr int kWords = sizeof(Status) / 4;
+        constexpr int kPub = offsetof(Status, published) / 4;
+        static_assert(kWords <= kWave, "one wave publishes the status");
+        if (threadIdx.x < kWords && threadIdx.x != kPub) {
+            const int32_t v = reinterpret_cast<const int32_t *>(status)[threadIdx.x];
+            __hip_atomic_store(reinterpret_cast<int32_t *>(status_pub) + threadIdx.x, v, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");   // system scope: every lane's words land first
+        __builtin_amdgcn_wave_barrier();
+        if (threadIdx.x == 0)
+            __hip_atomic_store(&status_pub->published, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
     const Status st = *status;
     // the atoms owning this tile's first pair and the next tile's, read with

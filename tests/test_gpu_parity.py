@@ -378,3 +378,51 @@ def test_repeated_builds_replay_the_pipeline():
     nl = nlhip.neighbour_list(X, 5.0, s.cell, s.pbc, want_R=True, handle=h, no_graph=True)
     _same(nl, oracle.neighbour_list(X, 5.0, s.cell, s.pbc, want_R=True))
     assert h.graph_launches() == n0
+
+
+def test_replays_return_with_the_fill_in_flight():
+    """A replayed single build returns once its fill has published the build
+    status; the fill may still be writing pairs.  The pair count is already
+    final, and the device views wait for the fill: here they are read at
+    once with plain copies that do not follow the handle's stream.  The
+    positions alternate between two sets, written into one device array (the
+    captured pipeline reads the positions from a fixed address)."""
+    import ctypes as C
+    _lib = nlhip._lib
+    L = _lib.lib()
+    s = workloads.random_cubic(200_000, seed=5)
+    X2 = s.X + 0.3 * workloads.normal(77, s.X.size).reshape(s.X.shape)
+    refs = [oracle.neighbour_list(X, s.cutoff, s.cell, s.pbc) for X in (s.X, X2)]
+    Xs = [np.ascontiguousarray(X) for X in (s.X, X2)]
+    d = _lib.DeviceArray(Xs[0])
+    g = _lib.geometry(s.cell, s.cutoff, s.pbc, 4)
+    h = _lib.Handle()
+    n = C.c_int64()
+
+    def copy(ptr, shape, dtype):
+        out = np.empty(shape, dtype=dtype)
+        if out.nbytes:
+            _lib.check(L.nlh_memcpy(out.ctypes.data, ptr, out.nbytes), "memcpy")
+        return out
+
+    for k in range(12):
+        ref = refs[k % 2]
+        d.upload(Xs[k % 2])   # a blocking copy; the previous build reads no positions after it returns
+        n0 = h.graph_launches()
+        _lib.check(L.nlh_neighbour_list(h.ptr, d.ptr, s.X.shape[0], C.byref(g), _lib.NLH_X_DEVICE, C.byref(n)),
+                   "neighbour_list")
+        assert n.value == ref.npairs
+        if k in (5, 8, 11):
+            assert h.graph_launches() == n0 + 1   # this call replayed the pipeline
+            print(f"replay {k}: reading the views at once", flush=True)
+            v = _lib.PairListView()
+            _lib.check(L.nlh_pairlist_device(h.ptr, C.byref(v)), "pairlist_device")
+            P = int(v.npairs)
+            assert P == ref.npairs
+            assert np.array_equal(copy(v.first, s.X.shape[0] + 1, np.int32), ref.first)
+            assert np.array_equal(copy(v.i, P, np.int32), ref.i)
+            assert np.array_equal(copy(v.j, P, np.int32), ref.j)
+            assert np.array_equal(copy(v.S, (P, 3), np.int32), ref.S)
+    assert h.graph_launches() >= 9
+    h.close()
+    d.free()
