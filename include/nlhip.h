@@ -121,7 +121,7 @@ int nlh_materialize(nlh_handle *h, uint32_t flags, int64_t *npairs);
  * count, cell grid, positions buffer, capacities and flags; positions and
  * geometry values may differ) replays the pipeline captured as a HIP graph,
  * with identical results; NLH_NO_GRAPH runs the stages one by one.  A replay
- * returns once *npairs is final, while the last stage may still be writing
+ * (here or by nlh_batch_rebuild) returns once *npairs is final, while the last stage may still be writing
  * the pairs on the handle's stream: work queued on that stream, any later
  * nlh_* call on the handle, nlh_synchronize and nlh_device_synchronize see
  * the finished pair list. */

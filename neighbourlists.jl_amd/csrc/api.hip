@@ -136,6 +136,7 @@ struct nlh_handle {
     nlh::Status *status_host_dev = nullptr;   // its device address
     bool status_pub_pending = false;      // the last launched fill publishes the status
     bool inflight = false;                // a replay returned while its fill still writes pairs
+    bool offsets_stale = false;           // pair_offsets_host is fetched when first asked for
     bool graph_publishes = false;         // the captured pipeline ends with a fill
     void *staging = nullptr;              // pinned upload staging for system tables
     size_t staging_cap = 0;
@@ -815,6 +816,7 @@ void materialize(nlh_handle *h, uint32_t flags)
         h->materialized = true;
         h->have_R = want_R;
         h->pair_offsets_host.assign(h->nsys + 1, 0);
+        h->offsets_stale = false;
         return;
     }
     ensure_pair_outputs(h, std::max(h->pair_capacity, estimate_pairs(h)), want_R);
@@ -860,13 +862,19 @@ void finish_pairs(nlh_handle *h, bool want_R)
     h->have_R = want_R;
     // per-system pair offsets
     h->pair_offsets_host.resize(h->nsys + 1);
+    h->offsets_stale = false;
     if (h->nsys == 1) {
         h->pair_offsets_host[0] = 0;
         h->pair_offsets_host[1] = h->npairs;
+    } else if (h->inflight && !(h->ti_bytes == 4 && h->npairs >= INT32_MAX)) {
+        // a replay still filling: fetched by nlh_batch_pair_offsets once the
+        // stream is done (no system can exceed Int32 when the total does not)
+        h->offsets_stale = true;
     } else {
+        settle(h);
         HIPD(hipMemcpy(h->pair_offsets_host.data(), h->sys_pair_base.p, (h->nsys + 1) * 8, hipMemcpyDeviceToHost));
     }
-    if (h->ti_bytes == 4)
+    if (h->ti_bytes == 4 && !h->offsets_stale)
         for (int s = 0; s < h->nsys; ++s)
             if (h->pair_offsets_host[s + 1] - h->pair_offsets_host[s] >= INT32_MAX)
                 throw Error(NLH_EOVERFLOW, "pair count exceeds typemax(Int32); use int_type=Int64");
@@ -933,7 +941,6 @@ bool rebuild_graph(nlh_handle *h, bool want_R)
     check_positions(h);
     const nlh::Status &st = *h->status_host;
     if (st.npairs > h->pair_capacity || st.max_count > h->slot_cap || (st.wide && !h->wide)) return false;
-    if (h->nsys > 1) settle(h);   // a batch copies its per-system pair offsets back (a plain copy)
     finish_pairs(h, want_R);
     return true;
 }
@@ -1396,6 +1403,13 @@ int nlh_batch_pair_offsets(nlh_handle *h, int64_t *pair_offsets)
     return guarded([&] {
         if (!h || !pair_offsets) throw Error(NLH_EINVAL, "NULL argument");
         if (!h->materialized) throw Error(NLH_ESTATE, "no materialised pair list");
+        if (h->offsets_stale) {
+            HIPC(hipSetDevice(h->device));
+            settle(h);
+            HIPD(hipMemcpy(h->pair_offsets_host.data(), h->sys_pair_base.p, (h->nsys + 1) * 8,
+                           hipMemcpyDeviceToHost));
+            h->offsets_stale = false;
+        }
         std::memcpy(pair_offsets, h->pair_offsets_host.data(), (h->nsys + 1) * sizeof(int64_t));
     });
 }
