@@ -119,7 +119,7 @@ int nlh_materialize(nlh_handle *h, uint32_t flags, int64_t *npairs);
  * (src/cell_list.jl:897-916): build + materialise with a single wait.  A
  * call that repeats the previous call's shape on this handle (same atom
  * count, cell grid, positions buffer, capacities and flags; positions and
- * geometry values may differ) of fewer than 32,768 atoms replays the
+ * geometry values may differ) of fewer than 12,288 atoms replays the
  * pipeline captured as a HIP graph, with identical results; NLH_NO_GRAPH
  * runs the stages one by one.  Every build (and nlh_materialize,
  * nlh_batch_neighbour_list, nlh_batch_rebuild) returns once *npairs is

@@ -439,7 +439,7 @@ void choose_brick(const nlh_geometry &g, int64_t nat, int32_t budget, int32_t ha
 }
 
 // Repeated single builds replay the captured pipeline below this size.
-constexpr int64_t kReplayMaxAtoms = 32768;
+constexpr int64_t kReplayMaxAtoms = 12288;
 
 // A brick's LDS holds 4 BT halo atoms; the expected halo may fill 90% of it
 // (the rest absorbs the density's fluctuation: a brick that overflows is
@@ -1216,8 +1216,9 @@ int nlh_neighbour_list(nlh_handle *h, const double *X, int64_t nat, const nlh_ge
         // small builds are launch-bound and replay the graph; a larger one
         // launches its stages directly: returning at the fill's start, the
         // host has the next build's first kernels queued before the fill
-        // ends, sooner than a graph launch gets going (100k atoms: 90 us per
-        // build against 98 us replayed; 10k: 64 against 61 us)
+        // ends, sooner than a graph launch gets going (per build, direct
+        // against replayed: 100k atoms 90 / 98 us, 16k 58 / 63 us, 10k 64 /
+        // 61 us)
         const bool graph = repeat && !(flags & NLH_NO_GRAPH) && !h->profile && h->materialized && h->nat > 0 &&
                            h->nat < kReplayMaxAtoms;
         h->last_single = false;

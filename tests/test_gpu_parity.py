@@ -380,7 +380,7 @@ def test_repeated_builds_replay_the_pipeline():
     assert h.graph_launches() == n0
 
 
-@pytest.mark.parametrize("nat,replayed", [(200_000, False), (20_000, True)])
+@pytest.mark.parametrize("nat,replayed", [(200_000, False), (8_000, True)])
 def test_builds_return_with_the_fill_in_flight(nat, replayed):
     """A build returns once its fill has published the build status; the
     fill may still be writing pairs.  The pair count is already final, and
