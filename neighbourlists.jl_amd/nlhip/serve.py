@@ -130,8 +130,11 @@ class _Resident:
         self.rebuilds = 0
 
     # rebuilds before the pipeline is captured as a graph: capturing and
-    # instantiating costs milliseconds, more than a few eager rebuilds
-    GRAPH_AFTER = 3
+    # instantiating costs milliseconds, more than a few eager rebuilds; and a
+    # resident (at least small_atoms atoms) builds faster with its stages
+    # launched directly than replayed (the host queues the next build while
+    # the fill runs), so by default it is never captured
+    GRAPH_AFTER = 1 << 62
 
     def build(self, X: np.ndarray) -> int:
         """First build, or a rebuild on new positions (upload into the
