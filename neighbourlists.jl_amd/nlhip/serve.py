@@ -19,7 +19,7 @@ keeps the device busy:
 * residency: a large system keeps its positions and its cell list on the
   device in a handle of its own; a rebuild request uploads the new
   positions into the resident buffer (through the handle's pinned staging)
-  and replays the captured pipeline (nlh_batch_rebuild).  At most
+  and rebuilds on the resident's buffers (nlh_batch_rebuild).  At most
   `max_residents` systems stay resident: a new one then takes over the
   handle of the least recently used idle resident (device buffers only
   grow, so a steady stream allocates nothing; device frees synchronise the
@@ -138,8 +138,8 @@ class _Resident:
 
     def build(self, X: np.ndarray) -> int:
         """First build, or a rebuild on new positions (upload into the
-        resident buffer; eager for the first few rebuilds, then replays of
-        the captured pipeline)."""
+        resident buffer; stages launched directly for the first GRAPH_AFTER
+        rebuilds, replays of the captured pipeline after that)."""
         n = C.c_int64()
         X = np.ascontiguousarray(X, dtype=np.float64)
         if self.built:
