@@ -70,6 +70,14 @@ def _single(s, steps: int, warmup: int, profile: bool = False):
         step()
     _sync()
     wall = (time.perf_counter() - t0) / steps
+    # isolated builds: each one waits until its pair list is complete before
+    # the next starts (no overlap of a build's fill with the next call)
+    iso = []
+    for _ in range(max(5, min(steps, 20))):
+        t1 = time.perf_counter()
+        step()
+        h.synchronize()
+        iso.append(time.perf_counter() - t1)
     phases = None
     if profile:   # separate, event-instrumented runs for the per-kernel durations
         h.set_profiling(True)
@@ -82,7 +90,8 @@ def _single(s, steps: int, warmup: int, profile: bool = False):
         h.set_profiling(False)
         phases = {k: statistics.mean(v) for k, v in acc.items()}
     return {"npairs": int(n.value), "nat": int(X.shape[0]), "ms": wall * 1e3, "pairs_per_s": n.value / wall,
-            "ncells": int(g.ncells_total), "phases_ms": phases}
+            "ncells": int(g.ncells_total), "phases_ms": phases,
+            "isolated_ms": statistics.median(iso) * 1e3, "isolated_min_ms": min(iso) * 1e3}
 
 
 def _host_inclusive(s, steps: int) -> dict:
@@ -240,28 +249,60 @@ def _served(nreq: int, seed: int = 11) -> dict:
                            "speedup_systems_per_s": round(ss["systems_per_s"] / sp["systems_per_s"], 2)}}
 
 
-def _cpu_baseline(s) -> dict:
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_share() -> tuple[int, int, int]:
+    """(nproc: CPUs this process may run on, os.cpu_count(), threads used).
+    The threads are nproc capped at 16, the host-CPU share of one GPU on the
+    benchmark machines (their OMP_NUM_THREADS says the same)."""
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        nproc = os.cpu_count() or 1
+    cap = 16
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        cap = min(cap, int(env))
+    return nproc, os.cpu_count() or 1, max(1, min(nproc, cap))
+
+
+def _time_cpu(s, nt: int, samples: int) -> tuple[int, list]:
     import oracle
 
-    nt = min(16, os.cpu_count() or 1)
     oracle.time_neighbour_list(s.X, s.cutoff, s.cell, s.pbc, nt)   # warm-up
-    ts = []
-    P = 0
-    for _ in range(5):
+    ts, P = [], 0
+    for _ in range(samples):
         t0 = time.perf_counter()
         P = oracle.time_neighbour_list(s.X, s.cutoff, s.cell, s.pbc, nt)
         ts.append(time.perf_counter() - t0)
-    t1 = []
-    for _ in range(2):
-        t0 = time.perf_counter()
-        oracle.time_neighbour_list(s.X, s.cutoff, s.cell, s.pbc, 1)
-        t1.append(time.perf_counter() - t0)
+    return P, ts
+
+
+def _cpu_baseline(s) -> dict:
+    """The C restatement of the reference's CPU sort-based path (the
+    reference's own `julia -t N` cannot run here: no Julia), timed as the
+    reference's benchmark does (scripts/benchmark.jl:124-151: minimum of at
+    least 5 samples after a warm-up), at N threads and at one thread."""
+    nproc, ncpu, nt = _cpu_share()
+    P, ts = _time_cpu(s, nt, 5)
+    _, t1 = _time_cpu(s, 1, 5)
     return {"value": round(P / min(ts), 1), "unit": "pairs/s", "cores": nt, "kind": "port",
             "median_value": round(P / statistics.median(ts), 1),
-            "value_1_thread": round(P / min(t1), 1),
-            "sample": f"5 full builds of the 100k-atom cube after 1 warm-up (min reported); C restatement of the "
-                      f"reference CPU sort-based path, count and fill over {nt} threads; value_1_thread: 2 builds "
-                      f"on one thread (min)"}
+            "value_1_thread": round(P / min(t1), 1), "median_value_1_thread": round(P / statistics.median(t1), 1),
+            "nproc": nproc, "cpu_count": ncpu, "cpu_model": _cpu_model(),
+            "sample": f"the 100k-atom cube, 1 warm-up then 5 full builds at {nt} threads and 5 at one thread "
+                      f"(min reported, median beside it); C restatement of the reference CPU sort-based path "
+                      f"(serial binning, sort and offsets; count and fill over the threads); threads = nproc "
+                      f"({nproc}) capped at 16, one GPU's host-CPU share"}
 
 
 def _copy_calibration(nbytes: int, reps: int) -> dict:
@@ -339,6 +380,11 @@ def main():
                                "cutoff 5.0 A, pbc=(T,T,T), Float64, Int32 indices",
                    "npairs": head["npairs"], "ncells": head["ncells"]},
         "phases_ms_100k": {k: round(v, 4) for k, v in head["phases_ms"].items()},
+        "isolated_100k": {"ms": round(head["isolated_ms"], 4), "min_ms": round(head["isolated_min_ms"], 4),
+                          "pairs_per_s": round(head["npairs"] / (head["isolated_ms"] * 1e-3), 1),
+                          "note": "one build at a time, each waited for until its pair list is complete "
+                                  "(median); `value` is the pipelined rate of back-to-back builds, each "
+                                  "returning when its pair count is final"},
     }
     if not args.quick:
         out["host_inclusive_100k"] = _host_inclusive(c1, max(5, min(args.steps, 20)))

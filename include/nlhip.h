@@ -147,6 +147,19 @@ int nlh_copy_celllist(nlh_handle *h, double *X_sorted, void *perm, void *cell_id
 int nlh_neighbours(nlh_handle *h, int32_t sys, int64_t i, int64_t cap, void *j, double *R, void *S,
                    int64_t *count);
 
+/* Lazy access, a range of atoms: neighbours(clist, i) for every i in
+ * i_first .. i_first + n - 1 (1-based, local to system `sys`) in one pass
+ * on the device -- what a loop of for_each_neighbour / neighbours calls
+ * over the atoms returns (src/cell_list.jl:779-833), in the same order.
+ * offsets (host, n + 1 entries) receives the 0-based start of each atom's
+ * neighbours, offsets[n] their total; when offsets[n] <= cap the
+ * neighbours are written to the host buffers j (TI), R (3 x float64) and
+ * S (3 x TI), any of which may be NULL.  The results stay on the device
+ * until the next build or range query on the handle: asking again for the
+ * same range (e.g. with cap = offsets[n]) only copies them. */
+int nlh_neighbours_range(nlh_handle *h, int32_t sys, int64_t i_first, int64_t n, int64_t cap, int64_t *offsets,
+                         void *j, double *R, void *S);
+
 /* ---- batched build (many independent systems, one launch per stage) ----
  * Systems s = 0..nsys-1 own atoms [atom_offsets[s], atom_offsets[s+1]) of
  * X_batch; every geometry must share ti_bytes.  The reference has no

@@ -88,13 +88,18 @@ __device__ inline void wrap_and_shift(int32_t i, int32_t n, bool p, int32_t &w, 
     s = q;
 }
 
-/* position_to_cell_index + bin_wrap_and_shift (src/cell_list.jl:66-74, 141-147) */
+/* position_to_cell_index + bin_wrap_and_shift (src/cell_list.jl:66-74, 141-147).
+ * The reference's unsafe_trunc is undefined for values that do not fit the
+ * index type; here a cell index beyond +-1e9 (or NaN) becomes cell 1, the
+ * same value the library's binning uses -- a build rejects such positions
+ * (NLH_EINVAL), so a cell list that built never holds one. */
 template <typename TI>
 __device__ inline void cell_and_winding(const nlh_celllist_dev<TI> &c, const double *x, int32_t ci[3], int32_t w[3])
 {
     for (int k = 0; k < 3; ++k) {
         const double f = (c.inv_cell[0 + 3 * k] * x[0] + c.inv_cell[1 + 3 * k] * x[1]) + c.inv_cell[2 + 3 * k] * x[2];
-        const int32_t c0 = (int32_t)floor(f * (double)c.ncells[k] + 1.0);
+        const double v = floor(f * (double)c.ncells[k] + 1.0);
+        const int32_t c0 = (v > -1.0e9 && v < 1.0e9) ? (int32_t)v : 1;
         wrap_and_shift(c0, c.ncells[k], (c.pbc >> k) & 1, ci[k], w[k]);
     }
 }

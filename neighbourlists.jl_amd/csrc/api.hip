@@ -125,7 +125,15 @@ struct nlh_handle {
     hipStream_t stream = nullptr;
     DevBuf X, tables, key, slot, cell_count, bmat, recb, cell_start, tmp, rec, wbig, perm, cell_id,
         cell_offsets, pair_count, slots, slotsR, pair_off, tiles, lookback, sysmap, sys_pair_base, first, i, j, S, R, status, Xsorted, nb_j, nb_R,
-        nb_S, nb_count;
+        nb_S, nb_count, nr_count, nr_off, nr_j, nr_R, nr_S;
+    // lazy queries over a range of atoms (nlh_neighbours_range): the last
+    // range's results stay on the device, keyed by the build they came from
+    uint64_t gen = 0;                     // builds on this handle so far
+    bool nr_valid = false;
+    uint64_t nr_gen = 0;
+    int32_t nr_sys = 0;
+    int64_t nr_first = 0, nr_n = 0;
+    std::vector<int64_t> nr_off_host;     // n + 1 offsets of the last range
     // system tables, one device buffer: SysDev[nsys], atom bases int64[nsys + 1],
     // cell bases int32[nsys + 1], brick bases int32[nsys + 1]
     nlh::SysDev *sys_d = nullptr;
@@ -172,6 +180,7 @@ struct nlh_handle {
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
     bool graph_want_R = false;
+    bool graph_wide = false;             // slot-entry width the captured kernels were built for
     nlh::PipelineBufs graph_bufs;
     bool last_single = false;            // the previous call was a full single build ...
     uint32_t last_flags = 0;             // ... with these flags
@@ -631,6 +640,7 @@ void mark(nlh_handle *h, int k)
 
 void build_cells(nlh_handle *h)
 {
+    ++h->gen;
     mark(h, 0);
     nlh::launch_build(h->bufs(), h->stream);
     mark(h, 1);
@@ -926,6 +936,7 @@ void capture_graph(nlh_handle *h, bool want_R)
     HIPD(hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0));
     h->graph_bufs = b;
     h->graph_want_R = want_R;
+    h->graph_wide = h->wide;
 }
 
 // Rebuild through the captured graph.  Returns false when the pair outputs
@@ -940,9 +951,14 @@ bool rebuild_graph(nlh_handle *h, bool want_R)
     ensure_slots(h, std::max(h->slot_cap, estimate_slot_cap(h)), want_R);
     check_pair_buffers(h, want_R);
     const nlh::PipelineBufs b = h->bufs();
-    if (!h->graph_exec || h->graph_want_R != want_R || std::memcmp(&b, &h->graph_bufs, sizeof b) != 0)
+    // the slot-entry width is baked into the captured count and fill: a build
+    // that widened the entries (a shift beyond +-511) recaptures even when
+    // the regrown slot buffer came back at the same address
+    if (!h->graph_exec || h->graph_want_R != want_R || h->graph_wide != h->wide ||
+        std::memcmp(&b, &h->graph_bufs, sizeof b) != 0)
         capture_graph(h, want_R);
     if (h->graph_publishes) h->status_host->published = 0;   // the previous fill has published (or ended)
+    ++h->gen;
     HIPC(hipGraphLaunch(h->graph_exec, h->stream));
     h->pair_lookback_clean = false;   // the replay's pair scan used them
     h->status_pub_pending = h->graph_publishes;
@@ -953,7 +969,7 @@ bool rebuild_graph(nlh_handle *h, bool want_R)
     wait_status(h);
     check_positions(h);
     const nlh::Status &st = *h->status_host;
-    if (st.npairs > h->pair_capacity || st.max_count > h->slot_cap || (st.wide && !h->wide)) return false;
+    if (st.npairs > h->pair_capacity || st.max_count > h->slot_cap || (st.wide && !h->graph_wide)) return false;
     finish_pairs(h, want_R);
     return true;
 }
@@ -1016,6 +1032,86 @@ __global__ void neighbours_one_kernel(nlh_celllist_dev<TI> c, int64_t i, int64_t
     });
     *count = n;
 }
+
+// neighbours of a range of atoms: one thread per atom, a counting pass and
+// a filling pass at host-scanned offsets, both through the device-side
+// header's traversal (the reference's order, R and S bit for bit)
+template <typename TI>
+__global__ __launch_bounds__(256) void neighbours_count_kernel(nlh_celllist_dev<TI> c, int64_t i_first, int64_t n,
+                                                               int64_t *__restrict__ count)
+{
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    int64_t k = 0;
+    nlh_for_each_neighbour(c, i_first + a, [&](TI, const double *, const TI *) { ++k; });
+    count[a] = k;
+}
+
+template <typename TI>
+__global__ __launch_bounds__(256) void neighbours_fill_kernel(nlh_celllist_dev<TI> c, int64_t i_first, int64_t n,
+                                                              const int64_t *__restrict__ off, TI *__restrict__ j,
+                                                              double *__restrict__ R, TI *__restrict__ S)
+{
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    int64_t q = off[a];
+    nlh_for_each_neighbour(c, i_first + a, [&](TI jj, const double *RR, const TI *SS) {
+        j[q] = jj;
+        for (int k = 0; k < 3; ++k) {
+            R[3 * q + k] = RR[k];
+            S[3 * q + k] = SS[k];
+        }
+        ++q;
+    });
+}
+
+namespace {
+
+// Cell-list view of system `sys` of the last build (positions, perm and
+// cell offsets of that system; perm and cell_offsets are system-local).
+nlh_celllist_view system_view(const nlh_handle *h, int32_t sys)
+{
+    const nlh::SysDev &sd = h->sys_host[sys];
+    const int ti = h->ti_bytes;
+    nlh_celllist_view v{};
+    v.nat = sd.nat;
+    v.ncells_total = sd.M;
+    v.X_orig = h->X_dev + 3 * sd.atom_base;
+    v.perm = static_cast<char *>(h->perm.p) + (size_t)sd.atom_base * ti;
+    v.cell_offsets = static_cast<char *>(h->cell_offsets.p) + ((size_t)sd.cell_base + sys) * ti;
+    v.ti_bytes = ti;
+    return v;
+}
+
+template <typename TI>
+void neighbours_range_launch(nlh_handle *h, int32_t sys, int64_t i_first, int64_t n)
+{
+    const auto c = nlh_make_celllist_dev<TI>(system_view(h, sys), h->geoms[sys]);
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    h->nr_count.ensure(h->stream, (size_t)n * 8);
+    hipLaunchKernelGGL(neighbours_count_kernel<TI>, dim3(blocks), dim3(256), 0, h->stream, c, i_first, n,
+                       h->nr_count.as<int64_t>());
+    HIPC(hipGetLastError());
+    // exclusive scan of the counts on the host (one pass over n values)
+    h->nr_off_host.assign((size_t)n + 1, 0);
+    HIPC(hipMemcpyAsync(h->nr_off_host.data() + 1, h->nr_count.p, (size_t)n * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    for (int64_t a = 0; a < n; ++a) h->nr_off_host[a + 1] += h->nr_off_host[a];
+    const int64_t total = h->nr_off_host[n];
+    const size_t t1 = (size_t)std::max<int64_t>(total, 1), ti = sizeof(TI);
+    h->nr_off.ensure(h->stream, ((size_t)n + 1) * 8);
+    h->nr_j.ensure(h->stream, t1 * ti);
+    h->nr_R.ensure(h->stream, t1 * 24);
+    h->nr_S.ensure(h->stream, t1 * 3 * ti);
+    HIPC(hipMemcpyAsync(h->nr_off.p, h->nr_off_host.data(), ((size_t)n + 1) * 8, hipMemcpyHostToDevice, h->stream));
+    if (total > 0)
+        hipLaunchKernelGGL(neighbours_fill_kernel<TI>, dim3(blocks), dim3(256), 0, h->stream, c, i_first, n,
+                           h->nr_off.as<int64_t>(), h->nr_j.as<TI>(), h->nr_R.as<double>(), h->nr_S.as<TI>());
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(h->stream));
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -1125,7 +1221,8 @@ int nlh_destroy(nlh_handle *h)
                           &h->cell_count, &h->bmat,     &h->recb,       &h->cell_start, &h->tmp,    &h->rec,
                           &h->wbig,     &h->perm,       &h->cell_id,  &h->cell_offsets, &h->pair_count, &h->slots, &h->slotsR, &h->pair_off,   &h->tiles, &h->lookback, &h->sysmap,
                           &h->sys_pair_base, &h->first, &h->i,          &h->j,            &h->S,      &h->R,
-                          &h->status,   &h->Xsorted,   &h->nb_j,       &h->nb_R,         &h->nb_S,   &h->nb_count};
+                          &h->status,   &h->Xsorted,   &h->nb_j,       &h->nb_R,         &h->nb_S,   &h->nb_count,
+                          &h->nr_count, &h->nr_off,    &h->nr_j,       &h->nr_R,         &h->nr_S};
         for (DevBuf *b : bufs) b->release(h->stream);
         device_wide([&] {
             if (h->status_host) (void)hipHostFree(h->status_host);
@@ -1331,13 +1428,7 @@ int nlh_neighbours(nlh_handle *h, int32_t sys, int64_t i, int64_t cap, void *j, 
         h->nb_R.ensure(h->stream, c1 * 24);
         h->nb_S.ensure(h->stream, c1 * 3 * ti);
         h->nb_count.ensure(h->stream, 8);
-        nlh_celllist_view v{};
-        v.nat = sd.nat;
-        v.ncells_total = sd.M;
-        v.X_orig = h->X_dev + 3 * sd.atom_base;
-        v.perm = static_cast<char *>(h->perm.p) + (size_t)sd.atom_base * ti;
-        v.cell_offsets = static_cast<char *>(h->cell_offsets.p) + ((size_t)sd.cell_base + sys) * ti;
-        v.ti_bytes = ti;
+        const nlh_celllist_view v = system_view(h, sys);
         if (ti == 8) {
             auto c = nlh_make_celllist_dev<int64_t>(v, h->geoms[sys]);
             hipLaunchKernelGGL(neighbours_one_kernel<int64_t>, dim3(1), dim3(64), 0, h->stream, c, i, cap,
@@ -1360,6 +1451,46 @@ int nlh_neighbours(nlh_handle *h, int32_t sys, int64_t i, int64_t cap, void *j, 
             if (S) HIPD(hipMemcpy(S, h->nb_S.p, m * 3 * ti, hipMemcpyDeviceToHost));
         }
         if (count) *count = n;
+    });
+}
+
+int nlh_neighbours_range(nlh_handle *h, int32_t sys, int64_t i_first, int64_t n, int64_t cap, int64_t *offsets,
+                         void *j, double *R, void *S)
+{
+    return guarded([&] {
+        if (!h || !offsets) throw Error(NLH_EINVAL, "NULL argument");
+        if (!h->built) throw Error(NLH_ESTATE, "no cell list built");
+        if (sys < 0 || sys >= h->nsys) throw Error(NLH_EINVAL, "no such system");
+        const nlh::SysDev &sd = h->sys_host[sys];
+        if (n < 0 || i_first < 1 || i_first + n - 1 > sd.nat)
+            throw Error(NLH_EINVAL, "atom range out of bounds (1-based, within the system)");
+        if (cap < 0) throw Error(NLH_EINVAL, "cap < 0");
+        HIPC(hipSetDevice(h->device));
+        settle(h);
+        const bool cached = h->nr_valid && h->nr_gen == h->gen && h->nr_sys == sys && h->nr_first == i_first &&
+                            h->nr_n == n;
+        if (!cached) {
+            h->nr_valid = false;
+            if (n > 0) {
+                if (h->ti_bytes == 8) neighbours_range_launch<int64_t>(h, sys, i_first, n);
+                else neighbours_range_launch<int32_t>(h, sys, i_first, n);
+            } else {
+                h->nr_off_host.assign(1, 0);
+            }
+            h->nr_valid = true;
+            h->nr_gen = h->gen;
+            h->nr_sys = sys;
+            h->nr_first = i_first;
+            h->nr_n = n;
+        }
+        std::memcpy(offsets, h->nr_off_host.data(), ((size_t)n + 1) * sizeof(int64_t));
+        const int64_t total = h->nr_off_host[n];
+        if (total > 0 && total <= cap) {
+            const size_t ti = (size_t)h->ti_bytes;
+            if (j) HIPD(hipMemcpy(j, h->nr_j.p, (size_t)total * ti, hipMemcpyDeviceToHost));
+            if (R) HIPD(hipMemcpy(R, h->nr_R.p, (size_t)total * 24, hipMemcpyDeviceToHost));
+            if (S) HIPD(hipMemcpy(S, h->nr_S.p, (size_t)total * 3 * ti, hipMemcpyDeviceToHost));
+        }
     });
 }
 

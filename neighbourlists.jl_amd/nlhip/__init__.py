@@ -8,13 +8,14 @@ from ._lib import (NLHError, NLHInvalidError, NLHMemoryError, NLHOverflowError, 
                    init, lib)
 from .api import (Int32, Int64, PairList, SortedCellList, build_cell_list, count_neighbours, cutoff,
                   for_each_neighbour, materialize_pairlist, max_neighbours, max_neigs, maxneigs, neighbour_list,
-                  neighbours, neigs, neigss, nneigs, npairs, nsites, num_neighbours, pairs, sites)
+                  neighbours, neighbours_range, neigs, neigss, nneigs, npairs, nsites, num_neighbours, pairs,
+                  sites)
 from .batch import BatchBuilder, BatchResult, batch_neighbour_list
 
 __all__ = [
     "NLHError", "NLHInvalidError", "NLHMemoryError", "NLHOverflowError", "Geometry", "Handle", "geometry",
     "init", "lib", "Int32", "Int64", "PairList", "SortedCellList", "build_cell_list", "count_neighbours",
     "cutoff", "for_each_neighbour", "materialize_pairlist", "max_neighbours", "max_neigs", "maxneigs",
-    "neighbour_list", "neighbours", "neigs", "neigss", "nneigs", "npairs", "nsites", "num_neighbours", "pairs",
+    "neighbour_list", "neighbours", "neighbours_range", "neigs", "neigss", "nneigs", "npairs", "nsites", "num_neighbours", "pairs",
     "sites", "BatchBuilder", "BatchResult", "batch_neighbour_list",
 ]

@@ -108,7 +108,7 @@ EXPORTS = [
     "nlh_init", "nlh_create", "nlh_destroy", "nlh_last_error", "nlh_synchronize", "nlh_get_stream",
     "nlh_geometry_init", "nlh_build_cell_list", "nlh_materialize", "nlh_neighbour_list",
     "nlh_pairlist_device", "nlh_celllist_device", "nlh_copy_pairlist", "nlh_copy_celllist",
-    "nlh_neighbours", "nlh_batch_neighbour_list", "nlh_batch_pair_offsets", "nlh_batch_rebuild",
+    "nlh_neighbours", "nlh_neighbours_range", "nlh_batch_neighbour_list", "nlh_batch_pair_offsets", "nlh_batch_rebuild",
     "nlh_set_tuning", "nlh_get_tuning", "nlh_autotune",
     "nlh_set_profiling", "nlh_phase_times", "nlh_graph_launches", "nlh_device_malloc", "nlh_device_free", "nlh_memcpy", "nlh_device_synchronize",
     "nlh_version",
@@ -133,6 +133,7 @@ def _declare(L):
     L.nlh_copy_pairlist.argtypes = [hp, vp, vp, vp, vp, vp]
     L.nlh_copy_celllist.argtypes = [hp, vp, vp, vp, vp]
     L.nlh_neighbours.argtypes = [hp, C.c_int32, C.c_int64, C.c_int64, vp, vp, vp, i64p]
+    L.nlh_neighbours_range.argtypes = [hp, C.c_int32, C.c_int64, C.c_int64, C.c_int64, i64p, vp, vp, vp]
     L.nlh_batch_neighbour_list.argtypes = [hp, C.c_int32, vp, i64p, C.POINTER(Geometry), C.c_uint32, i64p]
     L.nlh_batch_pair_offsets.argtypes = [hp, i64p]
     L.nlh_batch_rebuild.argtypes = [hp, vp, C.c_uint32, i64p]

@@ -16,7 +16,7 @@ this module                                reference
 ``neigs``, ``neigss``, ``neighbours``      cell_list.jl:549-606, 821-833
 ``for_each_neighbour``, ``count_neighbours`` cell_list.jl:779-814
 ``num_neighbours``                         cell_list.jl:926-927
-``pairs``, ``sites``                       iterators.jl:1-40
+``pairs``, ``sites``                       iterators.jl:9-40
 =========================================  ===================================
 
 Positions ``X`` are an (nat, 3) float64 array: a numpy array (host in,
@@ -313,19 +313,56 @@ def neighbours(obj, i: int):
     return _lazy_neighbours(obj, i)
 
 
-def _lazy_neighbours(clist: SortedCellList, i: int, sys: int = 0):
+def neighbours_range(clist: SortedCellList, i_first: int, n: int):
+    """Neighbours of atoms i_first .. i_first + n - 1 (1-based) in one pass
+    on the device (nlh_neighbours_range): (offsets, j, R, S), where atom
+    i_first + a owns entries offsets[a] .. offsets[a + 1] - 1 of j, R, S --
+    exactly what neighbours(clist, i) returns for each of them, in the
+    reference order (cell_list.jl:779-833)."""
     h = clist._handle
     dt = clist.int_type
-    n = C.c_int64()
-    check(lib().nlh_neighbours(h.ptr, sys, int(i), 0, None, None, None, C.byref(n)), "neighbours")
-    cap = n.value
-    js = np.empty(cap, dtype=dt)
-    Rs = np.empty((cap, 3), dtype=np.float64)
-    Ss = np.empty((cap, 3), dtype=dt)
-    if cap:
-        check(lib().nlh_neighbours(h.ptr, sys, int(i), cap, js.ctypes.data, Rs.ctypes.data, Ss.ctypes.data,
-                                   C.byref(n)), "neighbours")
-    return js, Rs, Ss
+    n = int(n)
+    offs = np.zeros(n + 1, dtype=np.int64)
+    op = offs.ctypes.data_as(C.POINTER(C.c_int64))
+    check(lib().nlh_neighbours_range(h.ptr, 0, int(i_first), n, 0, op, None, None, None), "neighbours")
+    tot = int(offs[-1])
+    js = np.empty(tot, dtype=dt)
+    Rs = np.empty((tot, 3), dtype=np.float64)
+    Ss = np.empty((tot, 3), dtype=dt)
+    if tot:   # the results are still on the device: this call only copies them
+        check(lib().nlh_neighbours_range(h.ptr, 0, int(i_first), n, tot, op, js.ctypes.data, Rs.ctypes.data,
+                                         Ss.ctypes.data), "neighbours")
+    return offs, js, Rs, Ss
+
+
+_LAZY_BLOCK = 1 << 16    # atoms per cached range query
+_LAZY_BLOCKS_KEPT = 4
+
+
+def _lazy_block(clist: SortedCellList, i: int):
+    """(first atom, offsets, j, R, S) of the cached block of atoms holding
+    atom i: lazy queries run over blocks of 65,536 atoms at a time, so a
+    sweep over the atoms costs a few device passes, not one per atom."""
+    blocks = clist.__dict__.setdefault("_lazy_blocks", {})
+    b = (int(i) - 1) // _LAZY_BLOCK
+    hit = blocks.get(b)
+    if hit is None:
+        a0 = b * _LAZY_BLOCK + 1
+        n = min(_LAZY_BLOCK, clist.nsites() - a0 + 1)
+        hit = (a0,) + neighbours_range(clist, a0, n)
+        if len(blocks) >= _LAZY_BLOCKS_KEPT:
+            blocks.pop(next(iter(blocks)))
+        blocks[b] = hit
+    return hit
+
+
+def _lazy_neighbours(clist: SortedCellList, i: int):
+    i = int(i)
+    if i < 1 or i > clist.nsites():
+        raise _lib.NLHInvalidError(_lib.NLH_EINVAL, "atom index out of range (1-based)")
+    a0, offs, js, Rs, Ss = _lazy_block(clist, i)
+    lo, hi = int(offs[i - a0]), int(offs[i - a0 + 1])
+    return js[lo:hi], Rs[lo:hi], Ss[lo:hi]
 
 
 def for_each_neighbour(f: Callable, clist: SortedCellList, i: int) -> None:
@@ -337,9 +374,8 @@ def for_each_neighbour(f: Callable, clist: SortedCellList, i: int) -> None:
 
 
 def count_neighbours(clist: SortedCellList, i: int) -> int:
-    n = C.c_int64()
-    check(lib().nlh_neighbours(clist._handle.ptr, 0, int(i), 0, None, None, None, C.byref(n)), "count_neighbours")
-    return int(n.value)
+    """cell_list.jl:809-814"""
+    return int(_lazy_neighbours(clist, i)[0].shape[0])
 
 
 def num_neighbours(obj, i: int) -> int:
@@ -354,7 +390,7 @@ def num_neighbours(obj, i: int) -> int:
 # ---------------------------------------------------------------------------
 
 def pairs(nlist: PairList):
-    """Iterate (i, j, R) over all pairs (iterators.jl:57-70)."""
+    """Iterate (i, j, R) over all pairs (iterators.jl:9-23)."""
     I, J, S = _host(nlist.i), _host(nlist.j), _host(nlist.S)
     R = _getR(nlist.X, J, I, S, np.asarray(nlist.C))
     for q in range(I.shape[0]):
@@ -362,7 +398,7 @@ def pairs(nlist: PairList):
 
 
 def sites(nlist: PairList):
-    """Iterate (i, j, R) per site (iterators.jl:75-85)."""
+    """Iterate (i, j, R) per site (iterators.jl:27-40)."""
     for i in range(1, nlist.nsites() + 1):
         J, R = neigs(nlist, i)
         yield i, J, R

@@ -17,13 +17,21 @@ keeps the device busy:
   buffers); the C calls release the GIL, so several builds are on the
   device at once;
 * residency: a large system keeps its positions and its cell list on the
-  device in a handle of its own; a rebuild request uploads the new
-  positions into the resident buffer (through the handle's pinned staging)
-  and rebuilds on the resident's buffers (nlh_batch_rebuild).  At most
-  `max_residents` systems stay resident: a new one then takes over the
-  handle of the least recently used idle resident (device buffers only
-  grow, so a steady stream allocates nothing; device frees synchronise the
-  whole device, so they happen only under HBM-budget pressure).
+  device in a handle of its own; a rebuild request with the same atom
+  count, cell, pbc and cutoff uploads the new positions into the resident
+  buffer (through the handle's pinned staging) and rebuilds on the
+  resident's buffers (nlh_batch_rebuild); a request for the system with a
+  new cell (NPT, strain), pbc, cutoff or atom count is a fresh build on the
+  resident's handle with its own geometry.  At most `max_residents`
+  systems stay resident: a new one then takes over the handle of the least
+  recently used idle resident (device buffers only grow, so a steady
+  stream allocates nothing; device frees synchronise the whole device, so
+  they happen only under HBM-budget pressure).  A resident whose system
+  has a request admitted or running is never evicted or recycled.
+* latency: a completion records when its build call returned (`returned`:
+  the pair count is known, the last kernel may still be writing pairs) and
+  when the pair list is complete on the device (`end`, after waiting for
+  the worker's stream); `latency` is measured to `end`.
 
 Every result equals the single-system build of the same input bit for bit
 (the batch and replay paths emit in the reference traversal order), which
@@ -73,13 +81,25 @@ class Completion:
     npairs: int
     arrival: float
     start: float
-    end: float
+    end: float                # the pair list is complete on the device
     batch: int                # systems in the unit of work that served it
     result: PairList | None = None
+    returned: float = 0.0     # the build call returned (pair count known)
 
     @property
     def latency(self) -> float:
+        """Arrival to a complete pair list on the device."""
         return self.end - self.arrival
+
+    @property
+    def return_latency(self) -> float:
+        """Arrival to the build call's return (the fill may still run)."""
+        return self.returned - self.arrival
+
+
+def _geometry_key(cell, pbc, cutoff) -> tuple:
+    """What a resident's cell list depends on besides the positions."""
+    return (np.asarray(cell, dtype=np.float64).tobytes(), tuple(bool(b) for b in pbc), float(cutoff))
 
 
 def _cross(u, v):
@@ -115,10 +135,11 @@ class _Resident:
 
     def __init__(self, req: Request, int_type, recycled=None):
         self.system_id = req.system_id
+        self.ti = _lib.ti_bytes_of(int_type)
         self.nat = req.nat
-        self.geom = geometry(req.cell, req.cutoff, req.pbc, _lib.ti_bytes_of(int_type))
-        self.cell, self.pbc, self.cutoff = req.cell, req.pbc, req.cutoff
-        self.bytes = estimate_bytes(req.nat, req.cell, req.cutoff, _lib.ti_bytes_of(int_type))
+        self.geom = geometry(req.cell, req.cutoff, req.pbc, self.ti)
+        self.key = _geometry_key(req.cell, req.pbc, req.cutoff)
+        self.bytes = estimate_bytes(req.nat, req.cell, req.cutoff, self.ti)
         self.handle = None
         if recycled is not None:
             self.handle, held = recycled
@@ -136,21 +157,31 @@ class _Resident:
     # the fill runs), so by default it is never captured
     GRAPH_AFTER = 1 << 62
 
-    def build(self, X: np.ndarray) -> int:
+    def build(self, req: Request) -> int:
         """First build, or a rebuild on new positions (upload into the
         resident buffer; stages launched directly for the first GRAPH_AFTER
-        rebuilds, replays of the captured pipeline after that)."""
+        rebuilds, replays of the captured pipeline after that).  A request
+        whose atom count, cell, pbc or cutoff differ from the resident's is
+        a fresh build with its own geometry on the resident's handle."""
         n = C.c_int64()
-        X = np.ascontiguousarray(X, dtype=np.float64)
-        if self.built:
+        X = np.ascontiguousarray(req.X, dtype=np.float64)
+        key = _geometry_key(req.cell, req.pbc, req.cutoff)
+        if self.built and key == self.key and req.nat == self.nat:
             flags = _lib.NLH_NO_GRAPH if self.rebuilds < self.GRAPH_AFTER else 0
             self.rebuilds += 1
             check(lib().nlh_batch_rebuild(self.handle.ptr, X.ctypes.data, flags, C.byref(n)), "rebuild")
         else:
+            if key != self.key or req.nat != self.nat:
+                self.geom = geometry(req.cell, req.cutoff, req.pbc, self.ti)
+                self.key, self.nat, self.rebuilds = key, req.nat, 0
             check(lib().nlh_neighbour_list(self.handle.ptr, X.ctypes.data, self.nat, C.byref(self.geom), 0,
                                            C.byref(n)), "neighbour_list")
             self.built = True
         return int(n.value)
+
+    def synchronize(self):
+        """Wait until the last build's pair list is complete."""
+        check(lib().nlh_synchronize(self.handle.ptr), "synchronize")
 
     def detach(self):
         """Hand the handle over to a new resident."""
@@ -178,12 +209,12 @@ class Server:
                   grow-only buffers, reused from one unit of work to the next)
     small_atoms   systems below this size are batched together; larger ones
                   are resident (own handle, positions and cell list kept on
-                  the device, rebuilt by graph replay)
+                  the device, rebuilt on their own buffers)
     batch_atoms   atoms per batched build
     max_residents resident systems kept; beyond it a new one recycles the
                   least recently used idle resident's device buffers (a
-                  resident being rebuilt is not idle, so up to
-                  max_residents + workers - 1 can be held)
+                  resident with a request admitted or running is not idle,
+                  so up to max_residents + workers - 1 can be held)
     keep_results  copy each result back to the host (PairList) on completion
     """
 
@@ -211,6 +242,9 @@ class Server:
         # device buffers only grow, so a warm server allocates nothing
         self._builders: list[BatchBuilder | None] = [None] * self.nworkers
         self._est: dict[int, int] = {}   # id(request) -> estimate_bytes, for one serve() call
+        # systems with a request admitted or running (under _lock): their
+        # residents are neither evicted nor recycled
+        self._busy: set[int] = set()
         self.t0 = 0.0
 
     # ------------------------------------------------------------------ workers
@@ -242,10 +276,12 @@ class Server:
             res = self._resident_for(r, nbytes)
             nbytes = 0                       # the reservation became the resident's
             with res.lock:
-                P = res.build(r.X)
+                P = res.build(r)
+                ret = time.perf_counter() - self.t0
+                res.synchronize()            # the pair list is complete
                 out = res.result(r.X) if self.keep else None
             end = time.perf_counter() - self.t0
-            return [("ok", Completion(r.rid, r.system_id, P, r.arrival, start, end, 1, out), nbytes)]
+            return [("ok", Completion(r.rid, r.system_id, P, r.arrival, start, end, 1, out, ret), nbytes)]
         bb = self._builders[local["slot"]]
         if bb is None:
             bb = self._builders[local["slot"]] = BatchBuilder(self.int_type)
@@ -258,14 +294,19 @@ class Server:
             n = C.c_int64()
             check(lib().nlh_neighbour_list(bb.handle.ptr, X.ctypes.data, X.shape[0], C.byref(g), 0, C.byref(n)),
                   "neighbour_list")
+            ret = time.perf_counter() - self.t0
+            bb.handle.synchronize()          # the pair list is complete
             out = None
             if self.keep:
                 from .api import _collect_pairs
                 out = _collect_pairs(bb.handle, g, X, int(n.value), False, None)
             end = time.perf_counter() - self.t0
-            return [("ok", Completion(r.rid, r.system_id, int(n.value), r.arrival, start, end, 1, out), nbytes)]
+            return [("ok", Completion(r.rid, r.system_id, int(n.value), r.arrival, start, end, 1, out, ret),
+                     nbytes)]
         systems = [System(r.X, r.cell, r.pbc, r.cutoff) for r in reqs]
         bb.run(systems, copy=False)
+        ret = time.perf_counter() - self.t0
+        bb.handle.synchronize()              # every pair list of the batch is complete
         po = np.zeros(len(reqs) + 1, dtype=np.int64)
         check(lib().nlh_batch_pair_offsets(bb.handle.ptr, po.ctypes.data_as(C.POINTER(C.c_int64))))
         res = None
@@ -277,25 +318,28 @@ class Server:
         for s, r in enumerate(reqs):
             out = res.pairlist(s) if res is not None else None
             done.append(("ok", Completion(r.rid, r.system_id, int(po[s + 1] - po[s]), r.arrival, start, end,
-                                          len(reqs), out), nbytes if s == 0 else 0))
+                                          len(reqs), out, ret), nbytes if s == 0 else 0))
         return done
 
     def _resident_for(self, r: Request, reserved: int) -> _Resident:
-        """The resident entry of r's system; a new one takes over the bytes
-        reserved for it at admission."""
+        """The resident entry of r's system; the bytes reserved for it at
+        admission become the resident's (a new resident's estimate, or what
+        an existing one's buffers may grow by for a larger request)."""
         with self._lock:
             res = self.residents.get(r.system_id)
             if res is not None:
                 self.residents.move_to_end(r.system_id)
                 self.inflight_bytes -= reserved
-                if res.nat != r.nat:
-                    raise ValueError(f"rebuild of system {r.system_id} with a different atom count")
+                self.resident_bytes += reserved
+                res.bytes += reserved
                 return res
             # at the resident limit, the least recently used idle resident
             # hands its device buffers to the new one
             recycled = None
             if len(self.residents) >= self.max_residents:
                 for sid in list(self.residents):
+                    if sid in self._busy:
+                        continue             # admitted or running: not idle
                     old = self.residents[sid]
                     if old.lock.acquire(blocking=False):
                         try:
@@ -328,6 +372,8 @@ class Server:
             for sid in list(self.residents):
                 if self.inflight_bytes + self.resident_bytes + nbytes <= self.budget:
                     break
+                if sid in self._busy:
+                    continue                 # admitted or running: not idle
                 res = self.residents[sid]
                 if res.lock.acquire(blocking=False):
                     try:
@@ -360,7 +406,9 @@ class Server:
         k = 0
         n = len(order)
         self.t0 = time.perf_counter()
-        inflight_systems = set()
+        with self._lock:
+            self._busy = set()
+        inflight_systems = self._busy
 
         def complete(d):
             st, c, nb = d
@@ -368,7 +416,7 @@ class Server:
                 raise self._errors[0]
             with self._lock:
                 self.inflight_bytes -= nb       # completions release memory
-            inflight_systems.discard(c.system_id)
+                inflight_systems.discard(c.system_id)
             out.append(c)
 
         # no cyclic garbage collection while serving: a collection pass over a
@@ -436,15 +484,22 @@ class Server:
             if r.system_id in inflight_systems:
                 continue                     # a system is built by one worker at a time
             if r.nat >= self.small_atoms:
-                res = self.residents.get(r.system_id)
-                nb = 0 if res is not None else self._estimate(r)
+                with self._lock:
+                    # marked busy before the admission check, so that no
+                    # eviction in it takes this system's own resident
+                    inflight_systems.add(r.system_id)
+                    res = self.residents.get(r.system_id)
+                # an existing resident grows by what the new request needs
+                # beyond its buffers (a larger atom count or a denser cell)
+                nb = self._estimate(r) if res is None else max(self._estimate(r) - res.bytes, 0)
                 if not self._fits(nb):
+                    with self._lock:
+                        inflight_systems.discard(r.system_id)
                     return None
                 with self._lock:
                     self.inflight_bytes += nb
                     self.peak_bytes = max(self.peak_bytes, self.inflight_bytes + self.resident_bytes)
                 del waiting[idx]
-                inflight_systems.add(r.system_id)
                 return ("large", [r], nb)
             batch, atoms, nb = [], 0, 0
             rest = []
@@ -468,8 +523,8 @@ class Server:
             with self._lock:
                 self.inflight_bytes += nb
                 self.peak_bytes = max(self.peak_bytes, self.inflight_bytes + self.resident_bytes)
-            for q in batch:
-                inflight_systems.add(q.system_id)
+                for q in batch:
+                    inflight_systems.add(q.system_id)
             return ("batch", batch, nb)
         return None
 
@@ -518,20 +573,29 @@ def _plain_loop(h, ti, order, out, t0, n, realtime, keep_results):
         g = geometry(r.cell, r.cutoff, r.pbc, ti)
         X = np.ascontiguousarray(r.X, dtype=np.float64)
         check(lib().nlh_neighbour_list(h.ptr, X.ctypes.data, X.shape[0], C.byref(g), 0, C.byref(n)), "plain")
+        ret = time.perf_counter() - t0
+        h.synchronize()   # the pair list is complete
         res = None
         if keep_results:
             from .api import _collect_pairs
             res = _collect_pairs(h, g, X, int(n.value), False, None)
         end = time.perf_counter() - t0
-        out.append(Completion(r.rid, r.system_id, int(n.value), r.arrival, start, end, 1, res))
+        out.append(Completion(r.rid, r.system_id, int(n.value), r.arrival, start, end, 1, res, ret))
 
 
 def summarize(done: list[Completion], wall: float | None = None) -> dict:
+    """p50_ms / p99_ms: arrival to a complete pair list on the device;
+    p50_return_ms / p99_return_ms: arrival to the build call's return (pair
+    count known, the last kernel possibly still writing pairs)."""
     lat = np.array([c.latency for c in done]) * 1e3
+    ret = np.array([c.return_latency for c in done]) * 1e3
     t_end = max(c.end for c in done)
     t_first = min(c.arrival for c in done)
     span = wall if wall is not None else (t_end - t_first)
     pairs = sum(c.npairs for c in done)
     return {"requests": len(done), "span_s": round(span, 4), "systems_per_s": round(len(done) / span, 1),
             "pairs_per_s": round(pairs / span, 1), "p50_ms": round(float(np.percentile(lat, 50)), 3),
-            "p99_ms": round(float(np.percentile(lat, 99)), 3), "mean_batch": round(float(np.mean([c.batch for c in done])), 2)}
+            "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "p50_return_ms": round(float(np.percentile(ret, 50)), 3),
+            "p99_return_ms": round(float(np.percentile(ret, 99)), 3),
+            "mean_batch": round(float(np.mean([c.batch for c in done])), 2)}

@@ -144,3 +144,37 @@ def test_autotune_keeps_results():
     _check_oracle(after, systems, X0, offs, [0, 1, 64, 127])
     plain = bb.rebuild(X0)                      # the tuned batch without R
     _check_oracle(plain, systems, X0, offs, [0, 1, 64, 127], R=False)
+
+
+def test_replay_after_entries_widen():
+    """A replay whose shifts outgrow the packed slot entries (windings beyond
+    +-511 cells) is redone on the stage path with 16-byte entries; the next
+    replays recapture the pipeline for the wide entries instead of reusing
+    the narrow capture (the regrown slot buffer may come back at the same
+    address).  Batched rebuilds and repeated single builds both."""
+    systems = [workloads.random_cubic(3000, seed=131), workloads.random_cubic(1500, seed=132)]
+    X0, offs = pack_systems(systems)
+    bb = BatchBuilder()
+    bb.run(systems, want_R=True)
+    bb.rebuild(X0, want_R=True)                            # captured (narrow entries)
+    far = X0.copy()
+    L0 = systems[0].cell[0, 0]
+    far[0:3000:7, 0] += 1000.0 * L0                        # system 0: x windings of 1000
+    far[1:3000:11, 1] -= 777.0 * L0
+    for rep in range(3):
+        print(f"widened replay {rep}", flush=True)
+        g = bb.rebuild(far, want_R=True)
+        _check_oracle(g, systems, far, offs, [0, 1])
+    # repeated single builds of one shape replay below 12,288 atoms
+    s = systems[0]
+    h = _lib.Handle()
+    for rep in range(2):
+        nl = nlhip.neighbour_list(s.X, s.cutoff, s.cell, s.pbc, want_R=True, handle=h)
+        _same(nl, oracle.neighbour_list(s.X, s.cutoff, s.cell, s.pbc, want_R=True))
+    Xf = far[:3000]
+    for rep in range(3):
+        print(f"widened single build {rep}", flush=True)
+        nl = nlhip.neighbour_list(Xf, s.cutoff, s.cell, s.pbc, want_R=True, handle=h)
+        _same(nl, oracle.neighbour_list(Xf, s.cutoff, s.cell, s.pbc, want_R=True))
+    assert h.graph_launches() >= 2
+    h.close()

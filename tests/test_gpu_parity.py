@@ -431,3 +431,74 @@ def test_builds_return_with_the_fill_in_flight(nat, replayed):
         assert h.graph_launches() == 0
     h.close()
     d.free()
+
+
+def test_lazy_sweep_over_every_atom():
+    """for_each_neighbour / neighbours over all 100,000 atoms of a cell list
+    (cell_list.jl:779-833) through the range query: a few device passes,
+    under a second, each atom's neighbours bit for bit the oracle's slice of
+    the materialised list (j, S, R and the per-atom counts)."""
+    import time
+
+    s = workloads.random_cubic(100_000, density=0.05, seed=1, cutoff=5.0)
+    ref = oracle.neighbour_list(s.X, 5.0, s.cell, s.pbc, want_R=True, nthreads=os.cpu_count() or 1)
+    cl = nlhip.build_cell_list(s.X, 5.0, s.cell, s.pbc)
+    t0 = time.perf_counter()
+    offs, js, Rs, Ss = nlhip.neighbours_range(cl, 1, 100_000)
+    dt = time.perf_counter() - t0
+    print(f"lazy sweep of 100k atoms: {dt:.3f} s", flush=True)
+    assert dt < 1.0
+    assert np.array_equal(offs + 1, ref.first)
+    assert np.array_equal(js, ref.j) and np.array_equal(Ss, ref.S)
+    assert np.array_equal(Rs.view(np.int64), ref.R.view(np.int64))
+    # the per-atom Python API reads blocks of the same query
+    t0 = time.perf_counter()
+    n = 0
+    for i in range(1, 100_001):
+        n += nlhip.count_neighbours(cl, i)
+    print(f"count_neighbours over 100k atoms: {time.perf_counter() - t0:.3f} s", flush=True)
+    assert n == ref.npairs
+    for i in (1, 65536, 65537, 99_999, 100_000):
+        j, R, S = nlhip.neighbours(cl, i)
+        a, b = ref.first[i - 1] - 1, ref.first[i] - 1
+        assert np.array_equal(j, ref.j[a:b]) and np.array_equal(S, ref.S[a:b])
+        assert np.array_equal(R.view(np.int64), ref.R[a:b].view(np.int64))
+    with pytest.raises(nlhip.NLHInvalidError):
+        nlhip.neighbours(cl, 100_001)
+
+
+def test_lazy_range_in_a_batch():
+    """The range query on one system of a batch (system-local indices)."""
+    import ctypes as C
+
+    from nlhip import _lib
+    from nlhip.batch import BatchBuilder
+
+    systems = workloads.batch_systems(24, seed=151, nmax=700)
+    bb = BatchBuilder(np.int64)
+    bb.run(systems, copy=False)
+    L = _lib.lib()
+    for sidx in (0, 5, 23):
+        sysm = systems[sidx]
+        ref = oracle.neighbour_list(sysm.X, sysm.cutoff, sysm.cell, sysm.pbc, ti_bits=64, want_R=True)
+        n = sysm.nat
+        offs = np.zeros(n + 1, np.int64)
+        op = offs.ctypes.data_as(C.POINTER(C.c_int64))
+        _lib.check(L.nlh_neighbours_range(bb.handle.ptr, sidx, 1, n, 0, op, None, None, None))
+        P = int(offs[-1])
+        assert P == ref.npairs
+        j = np.empty(max(P, 1), np.int64)
+        S = np.empty((max(P, 1), 3), np.int64)
+        R = np.empty((max(P, 1), 3), np.float64)
+        _lib.check(L.nlh_neighbours_range(bb.handle.ptr, sidx, 1, n, P, op, j.ctypes.data, R.ctypes.data,
+                                          S.ctypes.data))
+        assert np.array_equal(offs + 1, ref.first)
+        assert np.array_equal(j[:P], ref.j) and np.array_equal(S[:P], ref.S)
+        assert np.array_equal(R[:P].view(np.int64), ref.R.view(np.int64))
+        # a sub-range, and the bounds check
+        k = max(n // 3, 1)
+        o2 = np.zeros(k + 1, np.int64)
+        _lib.check(L.nlh_neighbours_range(bb.handle.ptr, sidx, n - k + 1, k, 0,
+                                          o2.ctypes.data_as(C.POINTER(C.c_int64)), None, None, None))
+        assert np.array_equal(o2, ref.first[n - k:] - ref.first[n - k])
+    assert L.nlh_neighbours_range(bb.handle.ptr, 0, 2, systems[0].nat, 0, op, None, None, None) == _lib.NLH_EINVAL

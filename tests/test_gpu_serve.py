@@ -92,3 +92,46 @@ def test_request_larger_than_budget_is_refused():
     srv = serve.Server(hbm_budget=1 << 20)
     with pytest.raises(MemoryError):
         srv.serve(reqs, realtime=False)
+
+
+def test_resident_requests_with_new_geometry():
+    """Requests for a resident system that change its cell (strain), its
+    cutoff or its atom count are built with their own geometry, not the
+    resident's first one: every result equals the plain call and the
+    oracle."""
+    from nlhip.serve import Request
+
+    base = workloads.random_cubic(6000, density=0.05, seed=141)
+    rng = np.random.default_rng(142)
+    X0, C0 = base.X, base.cell
+    reqs = [Request(0, 0, X0, C0, (True, True, True), 5.0, False, 0.0)]
+    X1 = X0 + 0.1 * rng.standard_normal(X0.shape)
+    reqs.append(Request(1, 0, X1, C0, (True, True, True), 5.0, True, 0.0))           # plain rebuild
+    strain = np.array([[1.03, 0.02, 0.0], [0.0, 0.98, 0.01], [0.0, 0.0, 1.01]])
+    reqs.append(Request(2, 0, X1 @ strain, C0 @ strain, (True, True, True), 5.0, True, 0.0))   # new cell
+    reqs.append(Request(3, 0, X1 @ strain, C0 @ strain, (True, False, True), 4.2, True, 0.0))  # pbc, cutoff
+    reqs.append(Request(4, 0, X1[:5500], C0, (True, True, True), 5.0, True, 0.0))   # fewer atoms
+    reqs.append(Request(5, 0, X1[:5500] + 0.05, C0, (True, True, True), 5.0, True, 0.0))
+    srv = serve.Server(workers=2, small_atoms=2048, keep_results=True)
+    mixed = srv.serve(reqs, realtime=False)
+    srv.close()
+    alone = serve.serve_plain(reqs, realtime=False, keep_results=True)
+    for m, a, r in zip(mixed, alone, reqs):
+        print(f"request {r.rid} nat={r.nat} rc={r.cutoff}", flush=True)
+        ref = oracle.neighbour_list(r.X, r.cutoff, r.cell, r.pbc, want_R=False)
+        assert m.npairs == a.npairs == ref.npairs
+        _same(m.result, a.result)
+        assert np.array_equal(m.result.j, ref.j) and np.array_equal(m.result.S, ref.S)
+        assert np.array_equal(m.result.first, ref.first)
+
+
+def test_served_latency_covers_the_fill():
+    """A completion's latency runs to the complete pair list; the call-return
+    time is reported beside it and never later."""
+    reqs = _stream(40, seed=27, nmax=120000)
+    srv = serve.Server(workers=2, small_atoms=8192)
+    done = srv.serve(reqs, realtime=False)
+    srv.close()
+    assert all(c.end >= c.returned >= c.start for c in done)
+    s = serve.summarize(done)
+    assert s["p50_ms"] >= s["p50_return_ms"] and s["p99_ms"] >= s["p99_return_ms"]
