@@ -28,92 +28,10 @@ using StaticArrays: SVector, SMatrix
 import NeighbourLists
 using NeighbourLists: PairList, SortedCellList
 
-export hip_neighbour_list, hip_build_cell_list, hip_materialize_pairlist, HIPCellList, hip_neighbours
+export hip_neighbour_list, hip_build_cell_list, hip_materialize_pairlist, HIPCellList, hip_neighbours,
+       hip_neighbours_range
 
-const libnlhip = get(ENV, "NLHIP_LIB", joinpath(@__DIR__, "..", "nlhip", "libnlhip.so"))
-
-const NLH_OK, NLH_EINVAL, NLH_EOVERFLOW, NLH_ENOMEM, NLH_EDEVICE, NLH_ESTATE = 0, 1, 2, 3, 4, 5
-const NLH_X_DEVICE = UInt32(0x1)
-const NLH_WANT_R = UInt32(0x2)
-
-# nlh_geometry (include/nlhip.h): identical C layout, 248 bytes
-struct NLHGeometry
-    cell::NTuple{9,Float64}
-    inv_cell::NTuple{9,Float64}
-    lens::NTuple{3,Float64}
-    cutoff::Float64
-    cutoff_sq::Float64
-    ncells::NTuple{3,Int64}
-    nxyz::NTuple{3,Int64}
-    ncells_total::Int64
-    pbc::NTuple{3,UInt8}
-    zero_volume::UInt8
-    ti_bytes::Int32
-end
-
-struct NLHCellListView
-    nat::Int64
-    ncells_total::Int64
-    X_orig::Ptr{Float64}
-    perm::Ptr{Cvoid}
-    cell_id::Ptr{Cvoid}
-    cell_offsets::Ptr{Cvoid}
-    ti_bytes::Int32
-end
-
-last_error() = unsafe_string(ccall((:nlh_last_error, libnlhip), Cstring, ()))
-
-function check(status::Integer, what::AbstractString)
-    status == NLH_OK && return nothing
-    msg = string(what, ": ", last_error())
-    status == NLH_EOVERFLOW && throw(OverflowError(msg))   # cell_list.jl:655-659
-    status == NLH_EINVAL && throw(ArgumentError(msg))
-    status == NLH_ENOMEM && throw(OutOfMemoryError())
-    error(msg)
-end
-
-const _initialised = Ref(false)
-function init(device::Integer = 0)
-    check(ccall((:nlh_init, libnlhip), Cint, (Cint,), device), "nlh_init")
-    _initialised[] = true
-    nothing
-end
-
-"""Owns one `nlh_handle` (a HIP stream and its device buffers); freed by a finalizer."""
-mutable struct Handle
-    ptr::Ptr{Cvoid}
-    function Handle()
-        _initialised[] || init(0)
-        r = Ref{Ptr{Cvoid}}(C_NULL)
-        check(ccall((:nlh_create, libnlhip), Cint, (Ptr{Ptr{Cvoid}},), r), "nlh_create")
-        h = new(r[])
-        finalizer(h) do x
-            x.ptr == C_NULL || ccall((:nlh_destroy, libnlhip), Cint, (Ptr{Cvoid},), x.ptr)
-            x.ptr = C_NULL
-        end
-        return h
-    end
-end
-
-ti_bytes(::Type{Int32}) = Int32(4)
-ti_bytes(::Type{Int64}) = Int32(8)
-ti_bytes(T::Type) = throw(ArgumentError("int_type must be Int32 or Int64, got $T"))
-
-_pbc(pbc::Bool) = (UInt8(pbc), UInt8(pbc), UInt8(pbc))
-_pbc(pbc) = (UInt8(pbc[1]), UInt8(pbc[2]), UInt8(pbc[3]))
-
-"""Geometry via the library's host restatement of analyze_cell (cell_list.jl:152-170).
-The SMatrix is column-major, which is the ABI's cell layout."""
-function geometry(cell::AbstractMatrix, cutoff::Real, pbc, int_type::Type)
-    C = SMatrix{3,3,Float64}(cell)
-    g = Ref{NLHGeometry}()
-    p = collect(_pbc(pbc))
-    check(ccall((:nlh_geometry_init, libnlhip), Cint,
-                (Ptr{Float64}, Float64, Ptr{UInt8}, Int32, Ptr{NLHGeometry}),
-                collect(C), Float64(cutoff), p, ti_bytes(int_type), g), "geometry")
-    g[].zero_volume != 0 && @warn("Zero volume cell detected. Results may be unreliable.")
-    return g[]
-end
+include("nlhip_abi.jl")
 
 """Device-resident sort-based cell list (the handle keeps the HIP build)."""
 struct HIPCellList{TI}
@@ -192,6 +110,30 @@ function hip_neighbours(c::HIPCellList{TI}, i::Integer; system::Integer = 0) whe
               "neighbours")
     end
     return j, R, S
+end
+
+"""Neighbours of atoms i_first .. i_first + n - 1 (1-based) in one pass on
+the device (nlh_neighbours_range): `(offsets, j, R, S)` where atom
+`i_first + a - 1` owns entries `offsets[a] + 1 : offsets[a + 1]` -- what
+`neighbours(clist, i)` gives for each of them (cell_list.jl:779-833)."""
+function hip_neighbours_range(c::HIPCellList{TI}, i_first::Integer, n::Integer; system::Integer = 0) where {TI}
+    offs = zeros(Int64, n + 1)
+    check(ccall((:nlh_neighbours_range, libnlhip), Cint,
+                (Ptr{Cvoid}, Int32, Int64, Int64, Int64, Ptr{Int64}, Ptr{Cvoid}, Ptr{Float64}, Ptr{Cvoid}),
+                c.handle.ptr, system, i_first, n, 0, offs, C_NULL, C_NULL, C_NULL), "neighbours")
+    P = offs[end]
+    j = Vector{TI}(undef, P)
+    R = Vector{SVector{3,Float64}}(undef, P)
+    S = Vector{SVector{3,TI}}(undef, P)
+    if P > 0   # the results are still on the device: this call only copies them
+        GC.@preserve j R S begin
+            check(ccall((:nlh_neighbours_range, libnlhip), Cint,
+                        (Ptr{Cvoid}, Int32, Int64, Int64, Int64, Ptr{Int64}, Ptr{Cvoid}, Ptr{Float64}, Ptr{Cvoid}),
+                        c.handle.ptr, system, i_first, n, P, offs, pointer(j), Ptr{Float64}(pointer(R)),
+                        pointer(S)), "neighbours")
+        end
+    end
+    return offs, j, R, S
 end
 
 """

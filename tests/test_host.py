@@ -132,10 +132,71 @@ def test_no_cpu_fallback_without_device():
         nlhip.neighbour_list(np.zeros((2, 3)), 1.0, np.eye(3) * 5, (True,) * 3)
 
 
+def _header_arity():
+    src = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    out = {}
+    for name, args in re.findall(r"\b(?:int|const char \*)\s*(nlh_[a-z0-9_]+)\s*\(([^;]*?)\);", src, flags=re.S):
+        a = args.strip()
+        out[name] = 0 if a in ("", "void") else a.count(",") + 1
+    return out
+
+
+def _julia_ccalls(src):
+    """(symbol, number of argument types) of every ccall in Julia source."""
+    out = []
+    for m in re.finditer(r"ccall\(\(:(nlh_[a-z0-9_]+), libnlhip\),\s*\w+,\s*\(", src):
+        depth, k = 1, m.end()
+        while depth:
+            depth += {"(": 1, ")": -1}.get(src[k], 0)
+            k += 1
+        types = src[m.end():k - 1].strip()
+        n = 0 if not types else len([t for t in re.split(r",(?![^{]*\})", types) if t.strip()])
+        out.append((m.group(1), n))
+    return out
+
+
+def test_julia_ccalls_match_the_header():
+    """Every ccall in the Julia sources binds a declared entry point with the
+    header's number of arguments."""
+    arity = _header_arity()
+    jdir = os.path.join(REPO, "neighbourlists.jl_amd", "julia")
+    for f in ("NeighbourListsHIP.jl", "nlhip_abi.jl", os.path.join("ext", "NeighbourListsAMDGPUExt.jl")):
+        calls = _julia_ccalls(open(os.path.join(jdir, f)).read())
+        assert calls, f
+        for name, n in calls:
+            assert name in arity, (f, name)
+            assert n == arity[name], (f, name, n, arity[name])
+
+
+def test_julia_extension_plugs_into_the_reference_dispatch():
+    """The AMDGPU package extension adds methods at the reference's own
+    dispatch point (_build_sorted_celllist on a ROCVector with ROCBackend,
+    materialize_pairlist on a SortedCellList of ROCVectors), returns the
+    reference's types over device arrays made by similar(X, ...), and the
+    PairList aliases X (src/gpu_kernels.jl:347-362)."""
+    ext = open(os.path.join(REPO, "neighbourlists.jl_amd", "julia", "ext", "NeighbourListsAMDGPUExt.jl")).read()
+    assert re.search(r"function NeighbourLists\._build_sorted_celllist\(X::ROCVector\{SVec\{Float64\}\}, "
+                     r"cell::SMat\{Float64\},\s*pbc::SVec\{Bool\}, cutoff::Float64, ::Type\{TI\},\s*"
+                     r"backend::ROCBackend\)", ext)
+    assert re.search(r"function NeighbourLists\.materialize_pairlist\(clist::SortedCellList\{Float64,TI,AT,VI\};",
+                     ext)
+    assert "where {TI, AT <: ROCVector, VI}" in ext
+    called = {n for n, _ in _julia_ccalls(ext)}
+    assert {"nlh_build_cell_list", "nlh_copy_celllist", "nlh_materialize", "nlh_copy_pairlist",
+            "nlh_neighbours_range"} <= called
+    assert "NLH_X_DEVICE" in ext and "pointer(X)" in ext
+    for arr in ("i_arr = similar(clist.X_orig, TI, P)", "S_arr = similar(clist.X_orig, SVec{TI}, P)",
+                "perm = similar(X, TI, nat)"):
+        assert arr in ext
+    assert re.search(r"PairList\{Float64, TI, AT, typeof\(i_arr\), typeof\(S_arr\)\}\(clist\.X_orig,", ext)
+    assert "analyze_cell(cell, cutoff, TI)" in ext
+
+
 def test_julia_shim_binds_declared_symbols():
     """The Julia ccall shim binds only entry points the header declares, and
     its geometry struct has the C layout (no Julia here: checked as text)."""
-    jl = open(os.path.join(REPO, "neighbourlists.jl_amd", "julia", "NeighbourListsHIP.jl")).read()
+    jdir = os.path.join(REPO, "neighbourlists.jl_amd", "julia")
+    jl = open(os.path.join(jdir, "NeighbourListsHIP.jl")).read() + open(os.path.join(jdir, "nlhip_abi.jl")).read()
     called = set(re.findall(r"ccall\(\(:(nlh_[a-z0-9_]+),", jl))
     assert called and called <= set(_declared())
     assert {"nlh_geometry_init", "nlh_build_cell_list", "nlh_materialize", "nlh_copy_pairlist",
