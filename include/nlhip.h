@@ -183,6 +183,28 @@ int nlh_batch_pair_offsets(nlh_handle *h, int64_t *pair_offsets /* nsys + 1, hos
  * neighbour_list (src/cell_list.jl:897-916). */
 int nlh_batch_rebuild(nlh_handle *h, const double *X_batch, uint32_t flags, int64_t *total_pairs);
 
+/* The build of nlh_neighbour_list that allocates nothing.  No reference
+ * counterpart: it is for callers that cannot allocate device memory (code
+ * that runs under HIP graph capture elsewhere, tight serving loops,
+ * harnesses that forbid hipMalloc).  The caller passes the device positions
+ * X (nat x 3 float64) and device outputs i, j (pair_capacity TI), S
+ * (3 x pair_capacity TI), first (nat + 1 TI) and, with NLH_WANT_R, R
+ * (3 x pair_capacity float64); the results equal nlh_neighbour_list's bit
+ * for bit.  Every other buffer of the build lives in one static device
+ * array of 128 MiB per device (held from the first nlh_init or call on a
+ * device until the process ends): a build that needs more (about 220,000
+ * atoms at 0.05 atoms/A^3 and cutoff 5 A without R, 60,000 with R) fails
+ * with NLH_ENOMEM and a message saying so.  Work runs on `stream` (a
+ * hipStream_t; NULL for the default stream), which is waited for before the
+ * call returns.  When the list has more pairs than pair_capacity the call
+ * fails with NLH_ENOMEM, writes nothing to i, j, S, R and reports the pair
+ * count in *npairs (call again with larger outputs).  Calls share the
+ * device's array: they are serialised within this process, and calls from
+ * several processes on one GPU that overlap race. */
+int nlh_neighbour_list_noalloc(void *stream, const double *X, int64_t nat, const nlh_geometry *geom, uint32_t flags,
+                               int64_t pair_capacity, void *i, void *j, void *S, void *first, double *R,
+                               int64_t *npairs);
+
 /* Launch configuration of the build.  brick_atoms: largest expected atom
  * count (halo included) of one count-pass brick, which sets the brick shape
  * of each system (64..2048; default 920; at most 90% of the 4 x count_block

@@ -1113,6 +1113,248 @@ void neighbours_range_launch(nlh_handle *h, int32_t sys, int64_t i_first, int64_
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// The build that allocates nothing (nlh_neighbour_list_noalloc): every
+// buffer beyond the caller's inputs and outputs is carved out of one
+// file-scope device array.  Each device the library runs on holds its own
+// copy of the array, at its own address, until the process ends.
+// ---------------------------------------------------------------------------
+
+constexpr size_t kScratchBytes = (size_t)128 << 20;   // 128 MiB per device
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;                 // set-up, and one no-alloc build at a time per process
+void *g_scratch_ptr[kMaxDevices] = {};      // device address of `scratch` on each device
+int g_scratch_cus[kMaxDevices] = {};
+
+// The device address of `scratch` on `dev` (current).  The array's name in
+// host code is a host-side stand-in, not a device address: only this
+// pointer may reach kernels and copies.  Call with g_scratch_mutex held.
+void *scratch_ptr(int dev)
+{
+    if (dev < 0 || dev >= kMaxDevices) throw Error(NLH_EINVAL, "device index beyond the supported range");
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIPD(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw Error(NLH_EDEVICE, "hipGetSymbolAddress returned no address for the scratch array");
+        int cus = 0;
+        HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        g_scratch_cus[dev] = cus > 0 ? cus : 256;
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+// Bump allocation of 256-byte aligned pieces; with base == nullptr it only
+// measures.
+struct Carve {
+    char *base;
+    size_t used = 0;
+    template <typename T>
+    T *take(size_t bytes)
+    {
+        const size_t o = (used + 255) & ~(size_t)255;
+        used = o + std::max<size_t>(bytes, 8);
+        return reinterpret_cast<T *>(base ? base + o : nullptr);
+    }
+};
+
+struct NoAllocPlan {
+    nlh::SysDev sd;
+    int64_t atom_base[2];
+    int32_t cell_base[2];
+    int32_t brick_base[2];
+    int64_t nbricks;
+    nlh::SortPlan sort;
+};
+
+// Lay out every buffer of a single-system build in `base` (or only measure
+// when base is null).  The slot rows come last, so that growing them leaves
+// the cell list where the count pass reads it.  Returns the bytes used.
+size_t carve_noalloc(char *base, const NoAllocPlan &P, int ti, int32_t cap, bool wide, bool want_R, int64_t pair_capacity,
+                     const double *X, void *i_out, void *j_out, void *S_out, void *first_out, double *R_out,
+                     nlh::PipelineBufs &b)
+{
+    const size_t n1 = (size_t)std::max<int64_t>(P.sd.nat, 1);
+    const size_t M1 = (size_t)P.sd.M + 1;
+    Carve c{base};
+    std::memset(&b, 0, sizeof b);
+    b.X = X;
+    b.sys = c.take<nlh::SysDev>(sizeof(nlh::SysDev));
+    b.sys_atom_base = c.take<int64_t>(sizeof P.atom_base);
+    b.sys_cell_base = c.take<int32_t>(sizeof P.cell_base);
+    b.sys_brick_base = c.take<int32_t>(sizeof P.brick_base);
+    b.nsys = 1;
+    b.atom_sys = nullptr;
+    b.nbricks = P.nbricks;
+    b.nat = P.sd.nat;
+    b.M = P.sd.M;
+    b.sort = P.sort;
+    b.status = c.take<nlh::Status>(sizeof(nlh::Status));
+    b.status_pub = c.take<nlh::Status>(sizeof(nlh::Status));
+    b.key = c.take<int32_t>(n1 * 4);
+    b.slot = c.take<int32_t>(n1 * 4);
+    b.tmp = c.take<int32_t>(n1 * 4);
+    b.cell_count = c.take<int32_t>(M1 * 4 + 8);   // zeroed in whole 8-byte words
+    b.cell_start = c.take<int32_t>(M1 * 4);
+    b.bmat = c.take<int32_t>((size_t)std::max<int64_t>(P.sort.bw >= 0 ? P.sort.n_sort : 0, 1) * 4);
+    b.rec = c.take<nlh::Rec>(n1 * sizeof(nlh::Rec));
+    b.wbig = c.take<int32_t>(n1 * 12);
+    b.perm_out = c.take<void>(n1 * ti);
+    b.cell_id_out = c.take<void>(n1 * ti);
+    b.cell_offsets_out = c.take<void>(M1 * ti);
+    b.lookback = c.take<uint64_t>(
+        (size_t)nlh::lookback_entries(std::max<int64_t>((int64_t)M1, P.sort.n_sort), (int64_t)n1) * 8);
+    b.pair_count = c.take<int32_t>(n1 * 4);
+    b.pair_off = c.take<int64_t>((n1 + 1) * 8);
+    b.tile_cap = pair_capacity / nlh::kMinFillPairs + 2;
+    b.tile_atom = c.take<int32_t>((size_t)b.tile_cap * 8);
+    b.sys_pair_base = c.take<int64_t>(16);
+    // the tail: slot rows (the bucketed sort's (cell, index) scratch lives
+    // at their start until the count pass rewrites them), then their R
+    const size_t rows = n1 * (size_t)cap;
+    b.slots = c.take<void>(std::max(rows * (wide ? 16 : 8), n1 * nlh::kBucketBytes));
+    b.bke = reinterpret_cast<int2 *>(b.slots);
+    b.slot_cap = cap;
+    b.slotsR = want_R ? c.take<double>(rows * 24) : nullptr;
+    b.first_out = first_out;
+    b.i_out = i_out;
+    b.j_out = j_out;
+    b.S_out = S_out;
+    b.R_out = R_out;
+    b.pair_capacity = pair_capacity;
+    b.ti_bytes = ti;
+    b.fill_block = 256;
+    b.fill_pairs = 256;
+    b.count_block = 256;
+    return c.used;
+}
+
+// Expected neighbours per atom of one system, as a slot-row length.
+int32_t slot_cap_for(const nlh_geometry &g, int64_t nat)
+{
+    const double vol = std::fabs(det3(g.cell));
+    double mx = 0;
+    if (nat > 0 && vol > 0) mx = nat / vol * 4.18879020478639 * g.cutoff * g.cutoff * g.cutoff;
+    return round_slot_cap(std::min(mx * 1.5 + 16.0, 1.0e6));
+}
+
+// Wait for the stream and read the device status back.
+nlh::Status read_status(const nlh::PipelineBufs &b, hipStream_t st)
+{
+    nlh::Status s;
+    HIPC(hipMemcpyAsync(&s, b.status, sizeof s, hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    if (s.scan_error) throw Error(NLH_EDEVICE, "internal: a scan block waited too long for its predecessors");
+    if (s.bad_position) throw Error(NLH_EINVAL, "positions must be finite and within 1e9 cells of the box");
+    return s;
+}
+
+[[noreturn]] void scratch_too_small(size_t need)
+{
+    throw Error(NLH_ENOMEM, "this build needs " + std::to_string(need) + " bytes of scratch; the no-alloc entry point's "
+                            "static scratch holds " + std::to_string(kScratchBytes) +
+                            " bytes per device (nlh_neighbour_list has no such limit)");
+}
+
+void neighbour_list_noalloc(hipStream_t st, const double *X, int64_t nat, const nlh_geometry &g, uint32_t flags,
+                            int64_t pair_capacity, void *i_out, void *j_out, void *S_out, void *first_out,
+                            double *R_out, int64_t *npairs)
+{
+    validate_geometry(g);
+    const bool want_R = (flags & NLH_WANT_R) != 0;
+    if (nat < 0) throw Error(NLH_EINVAL, "nat < 0");
+    if (nat > INT32_MAX - 1 || (g.ti_bytes == 4 && nat >= INT32_MAX))
+        throw Error(NLH_EOVERFLOW, "too many atoms for one build");
+    if (pair_capacity < 0) throw Error(NLH_EINVAL, "pair_capacity < 0");
+    if (!first_out) throw Error(NLH_EINVAL, "first is NULL");
+    if (nat > 0 && !X) throw Error(NLH_EINVAL, "X is NULL");
+    if (pair_capacity > 0 && (!i_out || !j_out || !S_out || (want_R && !R_out)))
+        throw Error(NLH_EINVAL, "a pair output is NULL");
+    int dev = 0;
+    HIPC(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    char *base = static_cast<char *>(scratch_ptr(dev));
+    const int ti = g.ti_bytes;
+    if (nat == 0) {   // gpu_kernels.jl:303-312: first = [1]
+        const int64_t one64 = 1;
+        const int32_t one32 = 1;
+        HIPC(hipMemcpyAsync(first_out, ti == 8 ? (const void *)&one64 : (const void *)&one32, ti,
+                            hipMemcpyHostToDevice, st));
+        HIPC(hipStreamSynchronize(st));
+        *npairs = 0;
+        return;
+    }
+    // the system table (host side of the upload; the stream is waited for
+    // before it goes out of scope)
+    NoAllocPlan P{};
+    P.sd = make_sys(g, nat, 0, 0, 0);
+    const int cus = g_scratch_cus[dev];
+    choose_brick(g, nat, std::min<int32_t>(920, (int32_t)(kHaloFill * 4 * 256)), 256, P.sd.brick,
+                 2 * (int64_t)cus * 4);
+    P.sd.brick_base = 0;
+    P.nbricks = (int64_t)((P.sd.n[0] + P.sd.brick[0] - 1) / P.sd.brick[0]) *
+                ((P.sd.n[1] + P.sd.brick[1] - 1) / P.sd.brick[1]) * ((P.sd.n[2] + P.sd.brick[2] - 1) / P.sd.brick[2]);
+    if (P.nbricks > INT32_MAX - 1) throw Error(NLH_EOVERFLOW, "too many pair-search bricks");
+    P.atom_base[0] = 0;
+    P.atom_base[1] = nat;
+    P.cell_base[0] = 0;
+    P.cell_base[1] = P.sd.M;
+    P.brick_base[0] = 0;
+    P.brick_base[1] = (int32_t)P.nbricks;
+    P.sort = nlh::sort_plan(P.sd.M, nat, 0);
+    int32_t cap = slot_cap_for(g, nat);
+    bool wide = false;
+    nlh::PipelineBufs b;
+    size_t need = carve_noalloc(nullptr, P, ti, cap, wide, want_R, pair_capacity, X, i_out, j_out, S_out, first_out,
+                                R_out, b);
+    if (need > kScratchBytes) scratch_too_small(need);
+    carve_noalloc(base, P, ti, cap, wide, want_R, pair_capacity, X, i_out, j_out, S_out, first_out, R_out, b);
+    HIPC(hipMemcpyAsync((void *)b.sys, &P.sd, sizeof P.sd, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync((void *)b.sys_atom_base, P.atom_base, sizeof P.atom_base, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync((void *)b.sys_cell_base, P.cell_base, sizeof P.cell_base, hipMemcpyHostToDevice, st));
+    HIPC(hipMemcpyAsync((void *)b.sys_brick_base, P.brick_base, sizeof P.brick_base, hipMemcpyHostToDevice, st));
+    // the array keeps the previous call's contents: the status and the scan
+    // descriptors start at zero, and so does the atomic sort's histogram
+    nlh::launch_clear_status(b, st);
+    if (b.sort.bw < 0) nlh::launch_zero_words(reinterpret_cast<uint64_t *>(b.cell_count), ((int64_t)b.M + 2) / 2, st);
+    nlh::launch_build(b, st);
+    nlh::launch_count(b, want_R, wide, st);
+    nlh::launch_scan_pairs(b, st);
+    nlh::launch_fill(b, want_R, wide, st);
+    HIPC(hipGetLastError());
+    nlh::Status s = read_status(b, st);
+    if ((s.wide && !wide) || s.max_count > cap) {
+        // longer or wider slot rows, then count, scan and fill again (the
+        // cell list stays where it is: the slot rows are the layout's tail)
+        wide = wide || s.wide;
+        cap = std::max(cap, round_slot_cap(std::max(s.max_count, 1)));
+        need = carve_noalloc(nullptr, P, ti, cap, wide, want_R, pair_capacity, X, i_out, j_out, S_out, first_out,
+                             R_out, b);
+        if (need > kScratchBytes) scratch_too_small(need);
+        carve_noalloc(base, P, ti, cap, wide, want_R, pair_capacity, X, i_out, j_out, S_out, first_out, R_out, b);
+        nlh::launch_clear_pair_lookback(b, st);
+        nlh::launch_count(b, want_R, wide, st);
+        nlh::launch_scan_pairs(b, st);
+        nlh::launch_fill(b, want_R, wide, st);
+        HIPC(hipGetLastError());
+        s = read_status(b, st);
+        if ((s.wide && !wide) || s.max_count > cap)
+            throw Error(NLH_EDEVICE, "internal: slot rows still too short after growth");
+    }
+    *npairs = s.npairs;
+    if (ti == 4 && s.npairs >= INT32_MAX)
+        throw Error(NLH_EOVERFLOW, "pair count exceeds typemax(Int32); use int_type=Int64");
+    if (s.npairs > pair_capacity)
+        throw Error(NLH_ENOMEM, "the pair outputs hold " + std::to_string(pair_capacity) + " pairs, the list has " +
+                                    std::to_string(s.npairs) + " (*npairs): call again with larger outputs");
+}
+
+}  // namespace
+
 extern "C" {
 
 int nlh_device_malloc(uint64_t bytes, void **ptr)
@@ -1177,6 +1419,9 @@ int nlh_init(int device)
         HIPC(hipSetDevice(device));
         g_device = device;
         keep_pool_blocks(device);
+        // the no-alloc build's scratch: its device address on this device
+        std::lock_guard<std::mutex> lk(g_scratch_mutex);
+        (void)scratch_ptr(device);
     });
 }
 
@@ -1491,6 +1736,18 @@ int nlh_neighbours_range(nlh_handle *h, int32_t sys, int64_t i_first, int64_t n,
             if (R) HIPD(hipMemcpy(R, h->nr_R.p, (size_t)total * 24, hipMemcpyDeviceToHost));
             if (S) HIPD(hipMemcpy(S, h->nr_S.p, (size_t)total * 3 * ti, hipMemcpyDeviceToHost));
         }
+    });
+}
+
+int nlh_neighbour_list_noalloc(void *stream, const double *X, int64_t nat, const nlh_geometry *geom, uint32_t flags,
+                               int64_t pair_capacity, void *i, void *j, void *S, void *first, double *R,
+                               int64_t *npairs)
+{
+    return guarded([&] {
+        if (!geom || !npairs) throw Error(NLH_EINVAL, "NULL argument");
+        *npairs = 0;
+        neighbour_list_noalloc(static_cast<hipStream_t>(stream), X, nat, *geom, flags, pair_capacity, i, j, S, first,
+                               R, npairs);
     });
 }
 

@@ -719,6 +719,13 @@ void launch_clear_status(const PipelineBufs &b, hipStream_t st)
                        b.lookback, n);
 }
 
+void launch_zero_words(uint64_t *p, int64_t n, hipStream_t st)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(clear_status_kernel, dim3(std::min(blocks_for(n, 256), 256u)), dim3(256), 0, st,
+                       (Status *)nullptr, p, n);
+}
+
 void launch_clear_pair_lookback(const PipelineBufs &b, hipStream_t st)
 {
     const int64_t n = scan_tiles(b.nat) + 1;

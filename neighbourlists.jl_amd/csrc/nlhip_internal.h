@@ -267,6 +267,8 @@ struct PipelineBufs {
 };
 
 void launch_clear_status(const PipelineBufs &b, hipStream_t st);
+// Zero n 8-byte words by a kernel (no memset node: safe under graph capture).
+void launch_zero_words(uint64_t *p, int64_t n, hipStream_t st);
 void launch_clear_pair_lookback(const PipelineBufs &b, hipStream_t st);
 void launch_build(const PipelineBufs &b, hipStream_t st);
 void launch_count(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st);

@@ -109,7 +109,7 @@ EXPORTS = [
     "nlh_geometry_init", "nlh_build_cell_list", "nlh_materialize", "nlh_neighbour_list",
     "nlh_pairlist_device", "nlh_celllist_device", "nlh_copy_pairlist", "nlh_copy_celllist",
     "nlh_neighbours", "nlh_neighbours_range", "nlh_batch_neighbour_list", "nlh_batch_pair_offsets", "nlh_batch_rebuild",
-    "nlh_set_tuning", "nlh_get_tuning", "nlh_autotune",
+    "nlh_neighbour_list_noalloc", "nlh_set_tuning", "nlh_get_tuning", "nlh_autotune",
     "nlh_set_profiling", "nlh_phase_times", "nlh_graph_launches", "nlh_device_malloc", "nlh_device_free", "nlh_memcpy", "nlh_device_synchronize",
     "nlh_version",
 ]
@@ -137,6 +137,8 @@ def _declare(L):
     L.nlh_batch_neighbour_list.argtypes = [hp, C.c_int32, vp, i64p, C.POINTER(Geometry), C.c_uint32, i64p]
     L.nlh_batch_pair_offsets.argtypes = [hp, i64p]
     L.nlh_batch_rebuild.argtypes = [hp, vp, C.c_uint32, i64p]
+    L.nlh_neighbour_list_noalloc.argtypes = [vp, vp, C.c_int64, C.POINTER(Geometry), C.c_uint32, C.c_int64, vp, vp,
+                                             vp, vp, vp, i64p]
     L.nlh_set_tuning.argtypes = [hp, C.POINTER(Tuning)]
     L.nlh_get_tuning.argtypes = [hp, C.POINTER(Tuning)]
     L.nlh_autotune.argtypes = [hp, vp, C.c_uint32, C.c_int32, C.POINTER(Tuning)]
