@@ -215,7 +215,11 @@ int nlh_neighbour_list_noalloc(void *stream, const double *X, int64_t nat, const
  * 512, or 0 for the default (256).  cell_sort: 0 (default) sorts
  * the atoms by cell through buckets of cells counted in LDS wherever the
  * cell grid allows (up to 8192 buckets of 4096 cells), 1 uses one global
- * atomic histogram over the cells.
+ * atomic histogram over the cells.  fused_small: 0 (default) builds a batch
+ * of at least 64 systems, each with at most 2048 atoms, 2048 cells and a
+ * neighbour-cell reach of at most 3 per axis, by one workgroup per unit of
+ * consecutive systems (cell list, counts and slot rows in one launch, all in
+ * LDS); 1 always uses the general pipeline.
  * Every configuration gives identical results; no reference counterpart
  * (KernelAbstractions picks workgroup sizes itself). */
 typedef struct {
@@ -224,6 +228,7 @@ typedef struct {
     int32_t fill_pairs;
     int32_t count_block;
     int32_t cell_sort;
+    int32_t fused_small;
 } nlh_tuning;
 int nlh_set_tuning(nlh_handle *h, const nlh_tuning *t);
 int nlh_get_tuning(nlh_handle *h, nlh_tuning *t);

@@ -139,6 +139,12 @@ struct nlh_handle {
     nlh::SysDev *sys_d = nullptr;
     int64_t *atom_base_d = nullptr;
     int32_t *cell_base_d = nullptr, *brick_base_d = nullptr;
+    // fused build of small systems: units of consecutive systems (first
+    // system of each, then the end) and their launch order, in the tables
+    std::vector<int32_t> units_host, order_host;
+    int32_t nunits = 0;
+    int32_t *units_d = nullptr, *order_d = nullptr;
+    bool small_ok = true;   // cleared when a fused build met a winding beyond +-511
     std::vector<char> tables_host;        // bytes of the last upload
     nlh::Status *status_host = nullptr;   // pinned, mapped: the fill kernel writes it
     nlh::Status *status_host_dev = nullptr;   // its device address
@@ -187,7 +193,7 @@ struct nlh_handle {
     nlh::PipelineBufs last_bufs;         // ... on these buffers and sizes
     int64_t graph_launches = 0;
     // launch configuration (nlh_set_tuning / nlh_autotune)
-    nlh_tuning tune = {920, 256, 256, 0, 0};
+    nlh_tuning tune = {920, 256, 256, 0, 0, 0};
     int cus = 256;   // compute units of the device
 
     // count-pass workgroup: 4 waves per brick unless tuned (four 36 KB
@@ -248,6 +254,9 @@ struct nlh_handle {
         b.fill_block = tune.fill_block;
         b.fill_pairs = tune.fill_pairs;
         b.count_block = count_block();
+        b.nunits = nunits;
+        b.units = units_d;
+        b.unit_order = order_d;
         return b;
     }
 };
@@ -483,6 +492,50 @@ void assign_bricks(nlh_handle *h)
     h->nbricks = bricks;
 }
 
+// Fused build of small systems: a batch of at least kSmallMinSystems systems,
+// each within a unit's limits (atoms, cells, neighbour-cell reach), is cut
+// into units of consecutive systems, at most kUnitAtoms atoms, kUnitCells
+// cells and kUnitSystems systems each, launched largest first.  Otherwise
+// (or when turned off, or after a winding beyond +-511 made a fused build
+// fall back) no units: the general pipeline.
+constexpr int kSmallMinSystems = 64;
+
+void plan_units(nlh_handle *h)
+{
+    h->units_host.clear();
+    h->order_host.clear();
+    h->nunits = 0;
+    const int nsys = (int)h->sys_host.size();
+    if (!h->small_ok || h->wide || h->tune.fused_small != 0 || nsys < kSmallMinSystems) return;
+    int64_t total = 0;
+    for (const nlh::SysDev &sd : h->sys_host) {
+        total += sd.nat;
+        if (sd.nat > nlh::kUnitAtoms || sd.M > nlh::kUnitCells) return;
+        for (int k = 0; k < 3; ++k)
+            if (sd.nxyz[k] > nlh::kUnitMaxReach) return;
+    }
+    if (total == 0) return;   // no count stage follows: the general build writes the cell list
+    std::vector<int64_t> work;
+    int s = 0;
+    while (s < nsys) {
+        const int first = s;
+        int64_t atoms = 0, cells = 0;
+        while (s < nsys && s - first < nlh::kUnitSystems && atoms + h->sys_host[s].nat <= nlh::kUnitAtoms &&
+               cells + h->sys_host[s].M <= nlh::kUnitCells) {
+            atoms += h->sys_host[s].nat;
+            cells += h->sys_host[s].M;
+            ++s;
+        }
+        h->units_host.push_back(first);
+        work.push_back(atoms);
+    }
+    h->units_host.push_back(nsys);
+    h->nunits = (int32_t)work.size();
+    h->order_host.resize(h->nunits);
+    for (int u = 0; u < h->nunits; ++u) h->order_host[u] = u;
+    std::stable_sort(h->order_host.begin(), h->order_host.end(), [&](int a, int b) { return work[a] > work[b]; });
+}
+
 // Upload the system tables (sys, atom/cell/brick bases) as one copy through
 // pinned staging, unless the device already holds exactly these bytes (a
 // rebuild of the same systems).  Waits for earlier work on the stream before
@@ -493,19 +546,25 @@ void upload_tables(nlh_handle *h)
     const size_t sys_bytes = nsys * sizeof(nlh::SysDev);
     const size_t ab_bytes = (nsys + 1) * sizeof(int64_t);
     const size_t cb_bytes = (nsys + 1) * sizeof(int32_t);
+    const size_t un_bytes = h->units_host.size() * sizeof(int32_t), or_bytes = h->order_host.size() * sizeof(int32_t);
     const size_t o_ab = (sys_bytes + 15) / 16 * 16, o_cb = o_ab + (ab_bytes + 15) / 16 * 16,
-                 o_bb = o_cb + (cb_bytes + 15) / 16 * 16, total = o_bb + cb_bytes;
+                 o_bb = o_cb + (cb_bytes + 15) / 16 * 16, o_un = o_bb + (cb_bytes + 15) / 16 * 16,
+                 o_or = o_un + (un_bytes + 15) / 16 * 16, total = o_or + or_bytes;
     std::vector<char> img(total, 0);
     std::memcpy(img.data(), h->sys_host.data(), sys_bytes);
     std::memcpy(img.data() + o_ab, h->atom_base_host.data(), ab_bytes);
     std::memcpy(img.data() + o_cb, h->cell_base_host.data(), cb_bytes);
     std::memcpy(img.data() + o_bb, h->brick_base_host.data(), cb_bytes);
+    if (un_bytes) std::memcpy(img.data() + o_un, h->units_host.data(), un_bytes);
+    if (or_bytes) std::memcpy(img.data() + o_or, h->order_host.data(), or_bytes);
     const bool moved = h->tables.ensure(h->stream, total);
     char *base = static_cast<char *>(h->tables.p);
     h->sys_d = reinterpret_cast<nlh::SysDev *>(base);
     h->atom_base_d = reinterpret_cast<int64_t *>(base + o_ab);
     h->cell_base_d = reinterpret_cast<int32_t *>(base + o_cb);
     h->brick_base_d = reinterpret_cast<int32_t *>(base + o_bb);
+    h->units_d = h->nunits ? reinterpret_cast<int32_t *>(base + o_un) : nullptr;
+    h->order_d = h->nunits ? reinterpret_cast<int32_t *>(base + o_or) : nullptr;
     if (!moved && img == h->tables_host) return;
     HIPC(hipStreamSynchronize(h->stream));
     ensure_staging(h, total);
@@ -545,6 +604,7 @@ void setup_batch(nlh_handle *h, int32_t nsys, const int64_t *atom_offsets, const
     h->cell_base_host[nsys] = (int32_t)cells;
     h->nsys = nsys;   // before the bricks: the brick workgroup size depends on it
     assign_bricks(h);
+    plan_units(h);
     h->nat = nat;
     h->M = (int32_t)cells;
     h->ti_bytes = ti;
@@ -859,6 +919,17 @@ void materialize(nlh_handle *h, uint32_t flags)
     HIPC(hipGetLastError());
     wait_status(h);
     check_positions(h);
+    if (h->nunits > 0 && (h->status_host->small_fallback || h->status_host->wide)) {
+        // a winding beyond +-511 in a fused build: the general pipeline redoes it
+        settle(h);
+        h->small_ok = false;
+        plan_units(h);
+        upload_tables(h);
+        reset_status(h);
+        build_cells(h);
+        materialize(h, flags);
+        return;
+    }
     const nlh::Status &st = *h->status_host;
     if (st.npairs > h->pair_capacity || st.max_count > h->slot_cap || (st.wide && !h->wide)) {
         // grow the outputs / slots (or widen the slot entries) and repeat count + fill
@@ -968,6 +1039,13 @@ bool rebuild_graph(nlh_handle *h, bool want_R)
     h->ev_build = h->ev_pairs = false;
     wait_status(h);
     check_positions(h);
+    if (h->nunits > 0 && (h->status_host->small_fallback || h->status_host->wide)) {
+        settle(h);
+        h->small_ok = false;   // the general pipeline redoes the build (and is captured next time)
+        plan_units(h);
+        upload_tables(h);
+        return false;
+    }
     const nlh::Status &st = *h->status_host;
     if (st.npairs > h->pair_capacity || st.max_count > h->slot_cap || (st.wide && !h->graph_wide)) return false;
     finish_pairs(h, want_R);
@@ -994,15 +1072,19 @@ void validate_tuning(const nlh_tuning &t)
         throw Error(NLH_EINVAL, "count_block must be 0 (automatic), 256 or 512");
     if (t.cell_sort != 0 && t.cell_sort != 1)
         throw Error(NLH_EINVAL, "cell_sort must be 0 (bucketed where it applies) or 1 (atomic histogram)");
+    if (t.fused_small != 0 && t.fused_small != 1)
+        throw Error(NLH_EINVAL, "fused_small must be 0 (fused build of small systems where it applies) or 1 (off)");
 }
 
 void apply_tuning(nlh_handle *h, const nlh_tuning &t)
 {
     validate_tuning(t);
-    const bool bricks = t.brick_atoms != h->tune.brick_atoms || t.count_block != h->tune.count_block;
+    const bool bricks = t.brick_atoms != h->tune.brick_atoms || t.count_block != h->tune.count_block ||
+                        t.fused_small != h->tune.fused_small;
     h->tune = t;
     if (bricks && !h->sys_host.empty()) {
         assign_bricks(h);
+        plan_units(h);
         upload_tables(h);
     }
     drop_graph(h);
@@ -1881,7 +1963,7 @@ int nlh_autotune(nlh_handle *h, const double *X_batch, uint32_t flags, int32_t r
                 for (int32_t fb : fills)
                     for (int32_t fa : tiles) {
                     if (cb == 256 && ba > 920) continue;   // above what a 256-thread brick holds
-                    apply_tuning(h, nlh_tuning{ba, fb, fa, cb, start.cell_sort});
+                    apply_tuning(h, nlh_tuning{ba, fb, fa, cb, start.cell_sort, start.fused_small});
                     rebuild_eager(h, flags & ~NLH_NO_SYNC);   // warm-up (and buffer growth)
                     float tot = 0.0f;
                     for (int r = 0; r < reps; ++r) {

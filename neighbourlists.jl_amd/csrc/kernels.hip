@@ -772,6 +772,7 @@ static void launch_bucket_sort(const PipelineBufs &b, hipStream_t st)
 
 void launch_build(const PipelineBufs &b, hipStream_t st)
 {
+    if (b.nunits > 0) return;   // the fused build of small systems makes the cell list in the count stage
     if (b.nsys > 1)
         hipLaunchKernelGGL(sys_map_kernel, dim3(blocks_for(b.nat + b.M, 256)), dim3(256), 0, st, b.sys_atom_base,
                            b.sys_cell_base, b.nsys, b.nat, (int64_t)b.M, b.atom_sys);

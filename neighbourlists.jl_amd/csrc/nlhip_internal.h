@@ -145,6 +145,7 @@ struct Status {
     int32_t bad_position;    // non-finite / out-of-range position seen
     int32_t wide;            // a cell shift does not fit a packed slot entry
     int32_t max_count;       // largest per-atom neighbour count
+    int32_t small_fallback;  // the fused small-system build met a winding beyond +-511
 };
 
 // ---------------------------------------------------------------------------
@@ -261,6 +262,11 @@ struct PipelineBufs {
     Status *status;
     Status *status_pub;         // device address of the handle's pinned host copy of the status
     int ti_bytes;
+    // fused build of small systems (units of consecutive systems, one
+    // workgroup each): 0 units = the general pipeline
+    int32_t nunits = 0;
+    const int32_t *units = nullptr;       // nunits + 1: first system of each unit
+    const int32_t *unit_order = nullptr;  // nunits: launch order (largest unit first)
     int fill_block;          // threads per fill workgroup (multiple of 64)
     int fill_pairs;          // output pairs per fill wave (128, 256 or 512)
     int count_block;         // threads per count-pass brick workgroup (256 or 512)
@@ -277,5 +283,15 @@ void launch_scan_pairs(const PipelineBufs &b, hipStream_t st);
 // status to status_pub).
 bool launch_fill(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st);
 void launch_sorted_positions(const PipelineBufs &b, double *Xs, hipStream_t st);
+
+// Fused build of small systems (b.nunits > 0): the cell list of every
+// system, and with count = true also its pair counts and slot rows, by one
+// workgroup per unit of systems.
+constexpr int kSmallThreads = 1024;
+constexpr int kUnitAtoms = 2048;      // atoms of one unit (and of one system) at most
+constexpr int kUnitCells = 2048;      // cells of one unit at most
+constexpr int kUnitSystems = 256;     // systems of one unit at most
+constexpr int kUnitMaxReach = 3;      // neighbour-cell reach per axis at most
+void launch_small(const PipelineBufs &b, bool count, bool want_R, hipStream_t st);
 
 }  // namespace nlh

@@ -373,17 +373,29 @@ __device__ __forceinline__ int brick_cell_id(const Brick &B, int hc)
     return sd.cell_base + (B.b0[0] - 1 + ix) + (B.b0[1] - 1 + iy) * sd.n[0] + (B.b0[2] - 1 + iz) * sd.n[0] * sd.n[1];
 }
 
+// One halo atom in LDS, 32 bytes: a candidate is two 16-byte reads.
+struct __attribute__((aligned(16))) LAtom {
+    double x, y, z;
+    int32_t j;                      // j, or j | S code << 26 when every j fits in 26 bits
+    uint32_t t;                     // pack(s_halo - w_j); home cells have s_halo = 0
+};
+
+constexpr int kPassCands = kBrickChunks * kWave;   // candidates of one pass (three chunks)
+
 template <int BT>
 struct BrickLDS {
     using Sh = BrickShape<BT>;
-    // entry cap of x, y, z, j, t is a far-away atom that lanes past the last
-    // candidate read, so candidate loads need no per-lane branch
-    double x[Sh::cap + 1], y[Sh::cap + 1], z[Sh::cap + 1];
-    int32_t j[Sh::cap + 1];         // j, or j | S code << 26 when every j fits in 26 bits
-    uint32_t t[Sh::cap + 1];        // pack(s_halo - w_j); home cells have s_halo = 0
+    // entry cap is a far-away atom that lanes past the last candidate read,
+    // so candidate loads need no per-lane branch
+    LAtom a[Sh::cap + 1];
     int32_t hoff[Sh::halo + 1];     // LDS offset of each halo cell's atoms
     int32_t hstart[Sh::halo];       // first sorted index of each halo cell
     uint32_t hshift[Sh::halo];      // packed cell shift of each halo cell | kHomeCell
+    // per wave, for its current home cell: the LDS offset less the candidate
+    // prefix of each non-empty candidate row (in row order), and one byte per
+    // candidate of the current pass marking where a non-empty row starts
+    int32_t rowdl[Sh::waves][16];
+    uint32_t rowflag[Sh::waves][kPassCands / 4];
     int32_t total, flag;
     int32_t row8;                   // some shift needs 8-byte slot entries
 };
@@ -440,9 +452,9 @@ __device__ bool brick_load(const PairsArgs &A, const Brick &B, BrickLDS<BT> &L)
         L.total = (int32_t)tot;
         L.flag = 0;
         L.row8 = sd.nat > kNarrowMaxAtoms ? 1 : 0;
-        L.x[Sh::cap] = L.y[Sh::cap] = L.z[Sh::cap] = kFar;
-        L.j[Sh::cap] = 0;
-        L.t[Sh::cap] = kPackZero;
+        L.a[Sh::cap].x = L.a[Sh::cap].y = L.a[Sh::cap].z = kFar;
+        L.a[Sh::cap].j = 0;
+        L.a[Sh::cap].t = kPackZero;
     }
     __syncthreads();
     const int total = L.total;
@@ -489,11 +501,13 @@ __device__ bool brick_load(const PairsArgs &A, const Brick &B, BrickLDS<BT> &L)
             // the 10-bit biased fields of t carry the S codes of a 4-byte entry in
             // their low 2 bits (meaningful only when the brick stays 4-byte)
             const uint32_t code = (t & 3u) | ((t >> 8) & 12u) | ((t >> 16) & 48u);
-            L.t[idx] = t;
-            L.j[idx] = sd.nat <= kNarrowMaxAtoms ? (int32_t)((uint32_t)r[u].j | code << 26) : r[u].j;
-            L.x[idx] = r[u].x;
-            L.y[idx] = r[u].y;
-            L.z[idx] = r[u].z;
+            LAtom la;
+            la.x = r[u].x;
+            la.y = r[u].y;
+            la.z = r[u].z;
+            la.j = sd.nat <= kNarrowMaxAtoms ? (int32_t)((uint32_t)r[u].j | code << 26) : r[u].j;
+            la.t = t;
+            L.a[idx] = la;
         }
     }
     __syncthreads();
@@ -507,6 +521,7 @@ struct HomeRows {
     int h0, h1;      // LDS range of the home atoms (uniform)
     int pbase;       // sorted index of the first home atom (uniform)
     int rstart, rex; // per lane: LDS start and exclusive candidate prefix of row `lane`
+    int len;         // per lane: candidates in row `lane`
     int total;       // candidates (uniform)
     int self0;       // candidate index of home atom 0 (uniform)
 };
@@ -537,38 +552,66 @@ __device__ __forceinline__ HomeRows brick_home(const Brick &B, const BrickLDS<BT
     }
     R.rex = incl - len;
     R.rstart = st;
+    R.len = len;
     R.total = __builtin_amdgcn_readlane(incl, 8);
     // row 4 is (dz, dy) = (0, 0); the home cell is its middle cell
     R.self0 = __builtin_amdgcn_readlane(R.rex, 4) + (R.h0 - __builtin_amdgcn_readlane(R.rstart, 4));
     return R;
 }
 
-// LDS index of candidate k of a home cell (k < total): k plus the offset of
-// its row, the last row whose exclusive prefix is <= k.  The row is found by
-// counting the (wave-uniform) row prefixes at or below k -- an empty row
-// shares its prefix with the next row, which then wins -- and its offset
-// (LDS start - prefix, held by lane r for row r) is fetched by a lane permute,
-// which must run with lanes 0..8 active (an inactive source lane reads 0).
-struct RowTable {
-    int rex[9];   // exclusive candidate prefix of each row (uniform)
-    int dl;       // lane r < 9: LDS start of row r minus its prefix
+// LDS index of candidate k of a home cell (k < total): k plus the offset
+// (LDS start - candidate prefix) of the last non-empty row whose first
+// candidate is at or before k.  Per home cell the wave publishes the offsets
+// of its non-empty rows in row order (rowdl[q], q = the row's rank among the
+// non-empty rows).  Per pass of kPassCands candidates it marks with a byte
+// each candidate where a non-empty row other than the first starts and turns
+// the marks into one 64-bit word per chunk (a ballot); a candidate's rank q
+// is then the number of marks at or below it: those of the chunks before (a
+// popcount) and those of its own chunk up to its lane (mbcnt).  The row
+// maps are the wave's own LDS: written and read by the wave alone, in
+// program order.
+struct RowMap {
+    uint64_t sh[kBrickChunks];   // marks of each chunk, shifted right by one (uniform)
+    int base[kBrickChunks];      // marks before each chunk, plus the chunk's own first (uniform)
 };
 
-__device__ __forceinline__ RowTable row_table(const HomeRows &R)
+// Publish the row offsets of a home cell; returns this lane's rank among the
+// non-empty rows (meaningful on the lanes of non-empty rows, `ne`).
+template <int BT>
+__device__ __forceinline__ int row_publish(const HomeRows &R, BrickLDS<BT> &L, int wv, bool &ne)
 {
-    RowTable T;
-#pragma unroll
-    for (int r = 0; r < 9; ++r) T.rex[r] = __builtin_amdgcn_readlane(R.rex, r);
-    T.dl = R.rstart - R.rex;
-    return T;
+    const int lane = lane_id();
+    ne = lane < 9 && R.len > 0;
+    const int q = (int)mbcnt64(__builtin_amdgcn_ballot_w64(ne));
+    if (ne) L.rowdl[wv][q] = R.rstart - R.rex;
+    return q;
 }
 
-__device__ __forceinline__ int row_index(const RowTable &T, int k)
+template <int BT>
+__device__ __forceinline__ RowMap row_map(const HomeRows &R, BrickLDS<BT> &L, int wv, int k0, int q, bool ne)
 {
-    int row = 0;
+    const int lane = lane_id();
+    uint8_t *fl = reinterpret_cast<uint8_t *>(L.rowflag[wv]);
+    if (lane < kPassCands / 4) L.rowflag[wv][lane] = 0u;
+    const int p = R.rex - k0;   // pass-relative first candidate of row `lane`
+    if (ne && q > 0 && p > 0 && p < kPassCands) fl[p] = 1;
+    int acc = (int)__popcll(__builtin_amdgcn_ballot_w64(ne && q > 0 && p <= 0));
+    RowMap M;
 #pragma unroll
-    for (int r = 1; r < 9; ++r) row += k >= T.rex[r] ? 1 : 0;
-    return k + __builtin_amdgcn_ds_bpermute(row << 2, T.dl);
+    for (int g = 0; g < kBrickChunks; ++g) {
+        const uint64_t b = __builtin_amdgcn_ballot_w64(fl[g * kWave + lane] != 0);
+        M.sh[g] = b >> 1;
+        M.base[g] = acc + (int)(b & 1u);
+        acc += (int)__popcll(b);
+    }
+    return M;
+}
+
+template <int BT>
+__device__ __forceinline__ int row_index(const RowMap &M, const BrickLDS<BT> &L, int wv, int g, int k)
+{
+    const int q = M.base[g] + (int)mbcnt64(M.sh[g]);
+    return k + L.rowdl[wv][q];
 }
 
 // First slot of atom e's row: both factors are below 2^31, so one 32 x 32 -> 64
@@ -589,7 +632,9 @@ __device__ __forceinline__ int64_t slot_row(int64_t e, int32_t cap)
 // entries: every lane stores, lanes without a hit at an offset past the
 // resource's end, and the hardware's range check drops those stores and any
 // entry at r >= cap.  No exec-mask or bound branches.
-constexpr uint32_t kDropOffset = 0x80000000u;   // beyond any row resource (cap * 4 < 2^31)
+// beyond any row resource (cap * 4 < 2^30); the bit pattern of 4.0f, an
+// inline constant, so the select that applies it needs no literal
+constexpr uint32_t kDropOffset = 0x40800000u;
 constexpr int32_t kRawBufferWord3 = 0x00020000; // raw 32-bit buffer access (GFX9 family)
 
 template <bool NARROW, bool WZERO, bool WANT_R, bool WIDE>
@@ -599,15 +644,19 @@ __device__ __forceinline__ void emit_row(void *__restrict__ row, double *__restr
 {
     const uint32_t n = (uint32_t)__popcll(m);
     if (NARROW && !WANT_R) {
-        if (n != 0) {
-            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(row, 0, (int)(cap * 4u), kRawBufferWord3);
-            const uint32_t rk =
-                __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-            const uint32_t off = __builtin_amdgcn_inverse_ballot_w64(m) ? (rk + (uint32_t)ca) << 2 : kDropOffset;
-            // j | S code << 26 (w_i = 0 in a 4-byte brick)
-            __builtin_amdgcn_raw_buffer_store_b32((uint32_t)ent, rs, (int)off, 0, 0);
-            ca += (int32_t)n;
-        }
+        // every lane stores, each lane without a hit at an offset past the
+        // resource's end (a select on the ballot, not an exec-mask region);
+        // an empty ballot skips it
+        if (n == 0) return;
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(row, 0, (int)(cap * 4u), kRawBufferWord3);
+        const uint32_t rk = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        uint32_t off;
+        // a plain select on the 64-bit ballot (the compiler would otherwise
+        // open an exec-mask region around the offset computation)
+        __asm__("v_cndmask_b32_e64 %0, 4.0, %1, %2" : "=v"(off) : "v"((rk + (uint32_t)ca) << 2), "s"(m));
+        // j | S code << 26 (w_i = 0 in a 4-byte brick)
+        __builtin_amdgcn_raw_buffer_store_b32((uint32_t)ent, rs, (int)off, 0, 0);
+        ca += (int32_t)n;
         return;
     }
     if (n != 0) {
@@ -736,6 +785,111 @@ __device__ __forceinline__ void test_home_pair(int nch, const double *cx, const 
     }
 }
 
+// One pass of a home cell's atoms over loaded candidate chunks: every home
+// atom's distance tests against the nch chunks (cx, cy, cz, ent: j |
+// pack(s - w_j) << 32, or j | S code << 26 in a 4-byte brick), its hits
+// appended to its slot row at its running count (lane a of cnt_l for home
+// atom a).  home: the nh home atoms in LDS; ht_l, hj_l: lane a holds
+// pack(-w_i) and j of home atom a; kpass: the self candidate of home atom 0
+// in this pass (home atom a's is kpass + a); Cm: the cell matrix.
+template <bool WANT_R, bool WIDE>
+__device__ __forceinline__ void pass_home_atoms(bool narrow, int nch, const double *cx, const double *cy,
+                                                const double *cz, const uint64_t *ent, const LAtom *home, int nh,
+                                                uint32_t ht_l, int32_t hj_l, int kpass, const double *Cm, double rc2,
+                                                int64_t atom_base, void *__restrict__ slots, double *__restrict__ slotsR,
+                                                int32_t cap, Status *__restrict__ status, int32_t &cnt_l)
+{
+    const int lane = lane_id();
+    double sx[kBrickChunks], sy[kBrickChunks], sz[kBrickChunks];
+    uint32_t wcur = 0xffffffffu;   // home winding the shifts were computed for
+    int32_t wix = 0, wiy = 0, wiz = 0;
+    // wave-uniform, per chunk: the lanes whose shift is nonzero (the
+    // shifts sx, sy, sz are set only where this is nonzero or R is written)
+    uint64_t csh[kBrickChunks];
+    int a = 0;
+    if (narrow && nh >= 2) {
+        // every home atom of a 4-byte brick has w_i = 0: the shifts of
+        // the candidates are their own cell shifts
+#pragma unroll
+        for (int g = 0; g < kBrickChunks; ++g) {
+            csh[g] = 0;
+            if (g < nch) {
+                const uint32_t t = (uint32_t)(ent[g] >> 32);
+                csh[g] = __builtin_amdgcn_ballot_w64(t != kPackZero);
+                if (WANT_R || csh[g] != 0) {
+                    int32_t cx_, cy_, cz_;
+                    unpack_wind(t, cx_, cy_, cz_);
+                    cell_shift(Cm, cx_, cy_, cz_, sx[g], sy[g], sz[g]);
+                }
+            }
+        }
+        wcur = kPackZero;
+        for (; a + 1 < nh; a += 2) {
+            // uniform LDS reads: every lane receives the home atom's coordinates
+            const LAtom h0 = home[a], h1 = home[a + 1];
+            const double h0x = h0.x, h0y = h0.y, h0z = h0.z;
+            const double h1x = h1.x, h1y = h1.y, h1z = h1.z;
+            int32_t ca0 = __builtin_amdgcn_readlane(cnt_l, a), ca1 = __builtin_amdgcn_readlane(cnt_l, a + 1);
+            const int64_t srow0 = slot_row(atom_base + __builtin_amdgcn_readlane(hj_l, a), cap);
+            const int64_t srow1 = slot_row(atom_base + __builtin_amdgcn_readlane(hj_l, a + 1), cap);
+            void *row0 = WIDE ? (void *)(reinterpret_cast<int4 *>(slots) + srow0)
+                              : (void *)(reinterpret_cast<uint64_t *>(slots) + srow0);
+            void *row1 = WIDE ? (void *)(reinterpret_cast<int4 *>(slots) + srow1)
+                              : (void *)(reinterpret_cast<uint64_t *>(slots) + srow1);
+            double *rrow0 = WANT_R ? slotsR + 3 * srow0 : nullptr;
+            double *rrow1 = WANT_R ? slotsR + 3 * srow1 : nullptr;
+            test_home_pair<WANT_R, WIDE>(nch, cx, cy, cz, sx, sy, sz, csh, ent, h0x, h0y, h0z, h1x, h1y,
+                                         h1z, rc2, kpass + a, row0, row1, rrow0, rrow1,
+                                         (uint32_t)cap, status, ca0, ca1);
+            cnt_l = __builtin_amdgcn_inverse_ballot_w64(1ull << a) ? ca0 : cnt_l;
+            cnt_l = __builtin_amdgcn_inverse_ballot_w64(2ull << a) ? ca1 : cnt_l;
+        }
+    }
+    for (; a < nh; ++a) {
+        const LAtom hm = home[a];
+        const double hx = hm.x, hy = hm.y, hz = hm.z;
+        const uint32_t ht = (uint32_t)__builtin_amdgcn_readlane((int)ht_l, a);
+        if (ht != wcur) {   // first home atom, or a new home winding
+            int32_t tx, ty, tz;
+            unpack_wind(ht, tx, ty, tz);
+            wix = -tx; wiy = -ty; wiz = -tz;
+#pragma unroll
+            for (int g = 0; g < kBrickChunks; ++g) {
+                csh[g] = 0;
+                if (g < nch) {
+                    int32_t cx_, cy_, cz_;
+                    unpack_wind((uint32_t)(ent[g] >> 32), cx_, cy_, cz_);
+                    cx_ += wix; cy_ += wiy; cz_ += wiz;
+                    csh[g] = __builtin_amdgcn_ballot_w64((cx_ | cy_ | cz_) != 0);
+                    if (WANT_R || csh[g] != 0) cell_shift(Cm, cx_, cy_, cz_, sx[g], sy[g], sz[g]);
+                }
+            }
+            wcur = ht;
+        }
+        const bool wzero = ht == kPackZero;   // the home atom lies inside the cell
+        int32_t ca = __builtin_amdgcn_readlane(cnt_l, a);
+        // slot rows are indexed by original atom index, so the fill reads them in order
+        const int64_t srow = slot_row(atom_base + __builtin_amdgcn_readlane(hj_l, a), cap);
+        void *row = WIDE ? (void *)(reinterpret_cast<int4 *>(slots) + srow)
+                         : (void *)(reinterpret_cast<uint64_t *>(slots) + srow);
+        double *rrow = WANT_R ? slotsR + 3 * srow : nullptr;
+        const int ks = kpass + a;   // the self candidate's place in this pass
+        if (narrow)   // every home atom of a 4-byte brick has w_i = 0
+            test_home_atom<true, true, WANT_R, WIDE>(nch, cx, cy, cz, sx, sy, sz, csh, ent, hx, hy, hz, rc2,
+                                                     ks, row, rrow, (uint32_t)cap, status, ca, wix, wiy,
+                                                     wiz);
+        else if (wzero)
+            test_home_atom<false, true, WANT_R, WIDE>(nch, cx, cy, cz, sx, sy, sz, csh, ent, hx, hy, hz, rc2,
+                                                      ks, row, rrow, (uint32_t)cap, status, ca, wix, wiy,
+                                                      wiz);
+        else
+            test_home_atom<false, false, WANT_R, WIDE>(nch, cx, cy, cz, sx, sy, sz, csh, ent, hx, hy, hz,
+                                                       rc2, ks, row, rrow, (uint32_t)cap, status, ca, wix,
+                                                       wiy, wiz);
+        cnt_l = lane == a ? ca : cnt_l;
+    }
+}
+
 // Brick pass of the count: home atoms stream through the wave as scalars
 // (read out of one-per-lane registers), candidates sit one per lane in
 // registers, chunk by chunk; each (home atom, chunk) is one FP64 distance
@@ -774,123 +928,321 @@ __global__ __launch_bounds__(BT) void count_kernel(
     const uint32_t jmask = narrow ? 0xffffffffu : hjmask;   // entry low word: j | code << 26, or j
     for (int hc = wv; hc < B.nhome; hc += BrickShape<BT>::waves) {
         const HomeRows R = brick_home(B, L, hc);
-        const RowTable T = row_table(R);
+        bool rne;
+        const int rq = row_publish(R, L, wv, rne);
         const int nha = R.h1 - R.h0;
         for (int hg = 0; hg < nha; hg += kWave) {
             const int nh = sgpr(min(kWave, nha - hg));
             const int hb = R.h0 + hg;        // LDS index of home atom hg (uniform)
             const int hl = hb + (lane < nh ? lane : 0);
-            const uint32_t ht_l = L.t[hl];   // pack(-w_i): a home cell's own shift is zero
-            const int32_t hj_l = (int32_t)((uint32_t)L.j[hl] & hjmask);
+            const uint32_t ht_l = L.a[hl].t;   // pack(-w_i): a home cell's own shift is zero
+            const int32_t hj_l = (int32_t)((uint32_t)L.a[hl].j & hjmask);
             const int kself = R.self0 + hg;  // candidate index of home atom hg + a is kself + a
             int32_t cnt_l = 0;                // lane a: running count of home atom hg + a
             for (int k0 = 0; k0 < R.total; k0 += kBrickChunks * kWave) {
                 const int nch = sgpr(min(kBrickChunks, (R.total - k0 + kWave - 1) / kWave));
+                const RowMap Mp = row_map(R, L, wv, k0, rq, rne);
                 double cx[kBrickChunks], cy[kBrickChunks], cz[kBrickChunks];
-                double sx[kBrickChunks], sy[kBrickChunks], sz[kBrickChunks];
                 uint64_t ent[kBrickChunks];   // j | pack(s - w_j) << 32
 #pragma unroll
                 for (int g = 0; g < kBrickChunks; ++g) {
-                    if (g < nch) {   // chunks past nch are never read
-                        const int k = k0 + g * kWave + lane;
-                        // every lane takes part in row_index's permute: a lane reads
-                        // only from lanes that are active
-                        const int lr = row_index(T, k);
-                        const int li = k < R.total ? lr : BrickShape<BT>::cap;   // else the far sentinel
-                        cx[g] = L.x[li];
-                        cy[g] = L.y[li];
-                        cz[g] = L.z[li];
-                        ent[g] = (uint64_t)((uint32_t)L.j[li] & jmask) | ((uint64_t)L.t[li] << 32);
-                    }
+                    // every chunk of the pass is loaded (no branch): lanes past the
+                    // last candidate read the far sentinel, and chunks past nch
+                    // are never tested
+                    const int k = k0 + g * kWave + lane;
+                    const int lr = row_index(Mp, L, wv, g, k);
+                    const int li = k < R.total ? lr : BrickShape<BT>::cap;   // else the far sentinel
+                    const LAtom c = L.a[li];
+                    cx[g] = c.x;
+                    cy[g] = c.y;
+                    cz[g] = c.z;
+                    ent[g] = (uint64_t)((uint32_t)c.j & jmask) | ((uint64_t)c.t << 32);
                 }
-                uint32_t wcur = 0xffffffffu;   // home winding the shifts were computed for
-                int32_t wix = 0, wiy = 0, wiz = 0;
-                // wave-uniform, per chunk: the lanes whose shift is nonzero (the
-                // shifts sx, sy, sz are set only where this is nonzero or R is written)
-                uint64_t csh[kBrickChunks];
-                int a = 0;
-                if (narrow && nh >= 2) {
-                    // every home atom of a 4-byte brick has w_i = 0: the shifts of
-                    // the candidates are their own cell shifts
-#pragma unroll
-                    for (int g = 0; g < kBrickChunks; ++g) {
-                        csh[g] = 0;
-                        if (g < nch) {
-                            const uint32_t t = (uint32_t)(ent[g] >> 32);
-                            csh[g] = __builtin_amdgcn_ballot_w64(t != kPackZero);
-                            if (WANT_R || csh[g] != 0) {
-                                int32_t cx_, cy_, cz_;
-                                unpack_wind(t, cx_, cy_, cz_);
-                                cell_shift(Csh, cx_, cy_, cz_, sx[g], sy[g], sz[g]);
-                            }
-                        }
-                    }
-                    wcur = kPackZero;
-                    for (; a + 1 < nh; a += 2) {
-                        // uniform LDS reads: every lane receives the home atom's coordinates
-                        const double h0x = L.x[hb + a], h0y = L.y[hb + a], h0z = L.z[hb + a];
-                        const double h1x = L.x[hb + a + 1], h1y = L.y[hb + a + 1], h1z = L.z[hb + a + 1];
-                        int32_t ca0 = __builtin_amdgcn_readlane(cnt_l, a), ca1 = __builtin_amdgcn_readlane(cnt_l, a + 1);
-                        const int64_t srow0 = slot_row(sd.atom_base + __builtin_amdgcn_readlane(hj_l, a), cap);
-                        const int64_t srow1 = slot_row(sd.atom_base + __builtin_amdgcn_readlane(hj_l, a + 1), cap);
-                        void *row0 = WIDE ? (void *)(reinterpret_cast<int4 *>(slots) + srow0)
-                                          : (void *)(reinterpret_cast<uint64_t *>(slots) + srow0);
-                        void *row1 = WIDE ? (void *)(reinterpret_cast<int4 *>(slots) + srow1)
-                                          : (void *)(reinterpret_cast<uint64_t *>(slots) + srow1);
-                        double *rrow0 = WANT_R ? slotsR + 3 * srow0 : nullptr;
-                        double *rrow1 = WANT_R ? slotsR + 3 * srow1 : nullptr;
-                        test_home_pair<WANT_R, WIDE>(nch, cx, cy, cz, sx, sy, sz, csh, ent, h0x, h0y, h0z, h1x, h1y,
-                                                     h1z, rc2, kself + a - k0, row0, row1, rrow0, rrow1,
-                                                     (uint32_t)cap, status, ca0, ca1);
-                        cnt_l = __builtin_amdgcn_inverse_ballot_w64(1ull << a) ? ca0 : cnt_l;
-                        cnt_l = __builtin_amdgcn_inverse_ballot_w64(2ull << a) ? ca1 : cnt_l;
-                    }
-                }
-                for (; a < nh; ++a) {
-                    const double hx = L.x[hb + a], hy = L.y[hb + a], hz = L.z[hb + a];
-                    const uint32_t ht = (uint32_t)__builtin_amdgcn_readlane((int)ht_l, a);
-                    if (ht != wcur) {   // first home atom, or a new home winding
-                        int32_t tx, ty, tz;
-                        unpack_wind(ht, tx, ty, tz);
-                        wix = -tx; wiy = -ty; wiz = -tz;
-#pragma unroll
-                        for (int g = 0; g < kBrickChunks; ++g) {
-                            csh[g] = 0;
-                            if (g < nch) {
-                                int32_t cx_, cy_, cz_;
-                                unpack_wind((uint32_t)(ent[g] >> 32), cx_, cy_, cz_);
-                                cx_ += wix; cy_ += wiy; cz_ += wiz;
-                                csh[g] = __builtin_amdgcn_ballot_w64((cx_ | cy_ | cz_) != 0);
-                                if (WANT_R || csh[g] != 0) cell_shift(Csh, cx_, cy_, cz_, sx[g], sy[g], sz[g]);
-                            }
-                        }
-                        wcur = ht;
-                    }
-                    const bool wzero = ht == kPackZero;   // the home atom lies inside the cell
-                    int32_t ca = __builtin_amdgcn_readlane(cnt_l, a);
-                    // slot rows are indexed by original atom index, so the fill reads them in order
-                    const int64_t srow = slot_row(sd.atom_base + __builtin_amdgcn_readlane(hj_l, a), cap);
-                    void *row = WIDE ? (void *)(reinterpret_cast<int4 *>(slots) + srow)
-                                     : (void *)(reinterpret_cast<uint64_t *>(slots) + srow);
-                    double *rrow = WANT_R ? slotsR + 3 * srow : nullptr;
-                    const int ks = kself + a - k0;   // the self candidate's place in this pass
-                    if (narrow)   // every home atom of a 4-byte brick has w_i = 0
-                        test_home_atom<true, true, WANT_R, WIDE>(nch, cx, cy, cz, sx, sy, sz, csh, ent, hx, hy, hz, rc2,
-                                                                 ks, row, rrow, (uint32_t)cap, status, ca, wix, wiy,
-                                                                 wiz);
-                    else if (wzero)
-                        test_home_atom<false, true, WANT_R, WIDE>(nch, cx, cy, cz, sx, sy, sz, csh, ent, hx, hy, hz, rc2,
-                                                                  ks, row, rrow, (uint32_t)cap, status, ca, wix, wiy,
-                                                                  wiz);
-                    else
-                        test_home_atom<false, false, WANT_R, WIDE>(nch, cx, cy, cz, sx, sy, sz, csh, ent, hx, hy, hz,
-                                                                   rc2, ks, row, rrow, (uint32_t)cap, status, ca, wix,
-                                                                   wiy, wiz);
-                    cnt_l = lane == a ? ca : cnt_l;
-                }
+                pass_home_atoms<WANT_R, WIDE>(narrow, nch, cx, cy, cz, ent, L.a + hb, nh, ht_l, hj_l, kself - k0, Csh,
+                                              rc2, sd.atom_base, slots, slotsR, cap, status, cnt_l);
             }
             if (lane < nh) pair_count[sd.atom_base + hj_l] = cnt_l | (narrow ? 0 : kRow8Bit);
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused build of small systems
+// ---------------------------------------------------------------------------
+//
+// One workgroup of kSmallThreads threads per unit of consecutive systems (at
+// most kUnitAtoms atoms, kUnitCells cells and kUnitSystems systems), all of
+// it in LDS: the cell and winding of every atom, a stable counting sort by
+// cell (ties in ascending original index, Julia's stable sortperm), the
+// cell-list outputs, and -- with COUNT -- every atom's neighbour count and
+// slot row, through the brick pass's tests and emission.  A home cell's
+// candidates are its (2 nxyz + 1)^3 neighbour cells in the reference order
+// (dz, dy, dx outer to inner, gpu_kernels.jl:69-71), each a segment of the
+// sorted atoms with its cell shift, taken 64 segments at a time.
+
+struct SmallSort {                 // LDS of the sort
+    int16_t key[kUnitAtoms];       // unit cell of each atom
+    int16_t grp[kUnitAtoms];       // atoms grouped by cell, in arrival order
+    int32_t fill[kUnitCells];      // per cell: atoms, then arrivals
+    uint32_t wk[kUnitAtoms];       // packed winding of each atom
+};
+struct SmallRows {                 // LDS of one wave in the count
+    int32_t dl[kWave];             // LDS offset less candidate prefix of each non-empty segment
+    uint32_t sh[kWave];            // packed cell shift of each non-empty segment
+    uint32_t flag[kPassCands / 4]; // one byte per candidate of a pass: a segment starts there
+};
+struct SmallLDS {
+    LAtom a[kUnitAtoms + 1];       // sorted atoms {x, y, z, system-local j, packed winding}; far sentinel last
+    int32_t cs[kUnitCells + 1];    // first sorted atom of each unit cell
+    int32_t sys_cell0[kUnitSystems + 1];
+    int32_t sys_atom0[kUnitSystems + 1];
+    int32_t sys_row8[kUnitSystems];   // 8-byte slot rows: some winding, or reach beyond one cell
+    union {
+        SmallSort s;
+        SmallRows r[kSmallThreads / kWave];
+    } u;
+};
+
+constexpr uint32_t kPackNeg = (1024u) | (1024u << 10) | (1024u << 20);   // pack(-w) = kPackNeg - pack(w)
+
+struct SmallArgs {
+    const double *__restrict__ X;
+    const SysDev *__restrict__ sys;
+    const int64_t *__restrict__ sys_atom_base;
+    const int32_t *__restrict__ sys_cell_base;
+    const int32_t *__restrict__ units;
+    const int32_t *__restrict__ order;
+    int32_t *__restrict__ atom_sys;
+    Rec *__restrict__ rec;
+    void *__restrict__ perm_out;
+    void *__restrict__ cell_id_out;
+    void *__restrict__ cell_offsets_out;
+    int32_t *__restrict__ pair_count;
+    void *__restrict__ slots;
+    double *__restrict__ slotsR;
+    int32_t cap;
+    Status *__restrict__ status;
+};
+
+template <bool COUNT, bool WANT_R, typename TI>
+__global__ __launch_bounds__(kSmallThreads) void small_kernel(SmallArgs A)
+{
+    __shared__ SmallLDS L;
+    const int tid = (int)threadIdx.x, lane = lane_id(), wv = sgpr(tid >> 6);
+    const int u = A.order[blockIdx.x];
+    const int s0 = A.units[u], ns = A.units[u + 1] - s0;
+    const int64_t A0 = A.sys_atom_base[s0];
+    const int nat_u = (int)(A.sys_atom_base[s0 + ns] - A0);
+    const int32_t C0 = A.sys_cell_base[s0];
+    const int M_u = A.sys_cell_base[s0 + ns] - C0;
+    for (int k = tid; k <= ns; k += kSmallThreads) {
+        L.sys_cell0[k] = A.sys_cell_base[s0 + k] - C0;
+        L.sys_atom0[k] = (int32_t)(A.sys_atom_base[s0 + k] - A0);
+        if (k < ns) {
+            const SysDev &sd = A.sys[s0 + k];
+            L.sys_row8[k] = (sd.nxyz[0] > 1 || sd.nxyz[1] > 1 || sd.nxyz[2] > 1) ? 1 : 0;
+        }
+    }
+    for (int c = tid; c < M_u; c += kSmallThreads) L.u.s.fill[c] = 0;
+    if (tid == 0) {
+        LAtom f;
+        f.x = f.y = f.z = kFar;
+        f.j = 0;
+        f.t = kPackZero;
+        L.a[kUnitAtoms] = f;
+    }
+    __syncthreads();
+    // cell and winding of every atom (position_to_cell_index + bin_wrap_and_shift,
+    // cell_list.jl:66-74, 141-147), atoms per cell
+    bool bad = false, big = false;
+    for (int i = tid; i < nat_u; i += kSmallThreads) {
+        const int k = find_sys(L.sys_atom0, ns, (int32_t)i);
+        const SysDev &sd = A.sys[s0 + k];
+        const int64_t g = A0 + i;
+        int32_t c[3], w[3];
+        bad |= !cell_and_winding(sd, A.X[3 * g], A.X[3 * g + 1], A.X[3 * g + 2], c, w);
+        const int key = L.sys_cell0[k] + (c[0] - 1) + (c[1] - 1) * sd.n[0] + (c[2] - 1) * sd.n[0] * sd.n[1];
+        const uint32_t wp = pack_wind(w[0], w[1], w[2]);
+        L.u.s.key[i] = (int16_t)key;
+        L.u.s.wk[i] = wp;
+        if (wp != kPackZero) L.sys_row8[k] = 1;
+        big |= (wp & kWindBig) != 0;
+        atomicAdd(&L.u.s.fill[key], 1);
+        A.atom_sys[g] = s0 + k;   // the fill kernel's map of atoms to systems
+    }
+    if (bad) atomicOr(&A.status->bad_position, 1);
+    if (big) atomicOr(&A.status->small_fallback, 1);   // the general pipeline redoes the build
+    __syncthreads();
+    {   // cell starts: exclusive scan of the counts, two cells per thread
+        const int c = 2 * tid;
+        const int v0 = c < M_u ? L.u.s.fill[c] : 0, v1 = c + 1 < M_u ? L.u.s.fill[c + 1] : 0;
+        int64_t tot;
+        const int ex = (int)block_excl_scan_n<kSmallThreads>(v0 + v1, tot);   // its barriers end every read of fill
+        if (c < M_u) L.cs[c] = ex;
+        if (c + 1 < M_u) L.cs[c + 1] = ex + v0;
+        if (tid == 0) L.cs[M_u] = (int32_t)tot;
+        if (c < M_u) L.u.s.fill[c] = 0;
+        if (c + 1 < M_u) L.u.s.fill[c + 1] = 0;
+    }
+    __syncthreads();
+    for (int i = tid; i < nat_u; i += kSmallThreads) {   // atoms grouped by cell
+        const int key = L.u.s.key[i];
+        L.u.s.grp[L.cs[key] + atomicAdd(&L.u.s.fill[key], 1)] = (int16_t)i;
+    }
+    __syncthreads();
+    // each atom's place: its cell's start plus the atoms of its cell with a
+    // smaller index (the stable order); the sorted record, perm and cell_id
+    for (int i = tid; i < nat_u; i += kSmallThreads) {
+        const int key = L.u.s.key[i];
+        const int st = L.cs[key], en = L.cs[key + 1];
+        int rank = 0;
+        for (int t = st; t < en; ++t) rank += L.u.s.grp[t] < i;
+        const int p = st + rank;
+        const int k = find_sys(L.sys_atom0, ns, (int32_t)i);
+        const int64_t g = A0 + i;
+        LAtom la;
+        la.x = A.X[3 * g];
+        la.y = A.X[3 * g + 1];
+        la.z = A.X[3 * g + 2];
+        la.j = i - L.sys_atom0[k];
+        la.t = L.u.s.wk[i];
+        L.a[p] = la;
+        Rec r;
+        r.x = la.x;
+        r.y = la.y;
+        r.z = la.z;
+        r.j = la.j;
+        r.w = la.t;
+        A.rec[A0 + p] = r;
+        store_ti<TI>(A.perm_out, A0 + p, (int64_t)la.j + 1);
+        store_ti<TI>(A.cell_id_out, A0 + p, (int64_t)(key - L.sys_cell0[k]) + 1);
+    }
+    // cell offsets: per system, 1-based, its cells then a trailing entry
+    for (int c = tid; c < M_u; c += kSmallThreads) {
+        const int k = find_sys(L.sys_cell0, ns, (int32_t)c);
+        const int64_t gc = (int64_t)C0 + c + s0 + k;
+        store_ti<TI>(A.cell_offsets_out, gc, (int64_t)(L.cs[c] - L.sys_atom0[k]) + 1);
+        if (c == L.sys_cell0[k + 1] - 1)
+            store_ti<TI>(A.cell_offsets_out, gc + 1, (int64_t)(L.sys_atom0[k + 1] - L.sys_atom0[k]) + 1);
+    }
+    if (!COUNT) return;
+    __syncthreads();   // the sorted atoms are complete; the sort's LDS now holds the waves' segment maps
+    SmallRows &Rw = L.u.r[wv];
+    uint8_t *fl = reinterpret_cast<uint8_t *>(Rw.flag);
+    for (int hc = wv; hc < M_u; hc += kSmallThreads / kWave) {
+        const int h0 = sgpr(L.cs[hc]), nha = sgpr(L.cs[hc + 1]) - h0;
+        if (nha == 0) continue;
+        const int k = sgpr(find_sys(L.sys_cell0, ns, (int32_t)hc));
+        const SysDev &sd = A.sys[s0 + k];
+        const int cb = sgpr(L.sys_cell0[k]);
+        const int n0 = sd.n[0], n1 = sd.n[1], n2 = sd.n[2];
+        const int lc = hc - cb;
+        const int ci0 = lc % n0 + 1, ci1 = (lc / n0) % n1 + 1, ci2 = lc / (n0 * n1) + 1;
+        const int nx0 = sd.nxyz[0], nx1 = sd.nxyz[1], nx2 = sd.nxyz[2];
+        const int sxn = 2 * nx0 + 1, syn = 2 * nx1 + 1;
+        const int nseg = sxn * syn * (2 * nx2 + 1);
+        const int center = (nx2 * syn + nx1) * sxn + nx0;   // (dx, dy, dz) = 0: the home cell, zero shift
+        const bool p0 = (sd.pbc & 1) != 0, p1 = (sd.pbc & 2) != 0, p2 = (sd.pbc & 4) != 0;
+        const bool narrow = sgpr(L.sys_row8[k]) == 0;   // every S in [-1, 1]^3: 4-byte slot entries
+        for (int hg = 0; hg < nha; hg += kWave) {
+            const int nh = sgpr(min(kWave, nha - hg));
+            const int hb = h0 + hg;
+            const int hl = hb + (lane < nh ? lane : 0);
+            const uint32_t ht_l = kPackNeg - L.a[hl].t;   // pack(-w_i)
+            const int32_t hj_l = L.a[hl].j;
+            int32_t cnt_l = 0;
+            for (int sb = 0; sb < nseg; sb += kWave) {
+                // lane: neighbour cell sb + lane in the reference order, its
+                // sorted atoms and cell shift (_is_cell_in_bounds,
+                // _get_neighbor_cell, gpu_kernels.jl:20-47)
+                const int gs = sb + lane;
+                int st = 0, len = 0;
+                uint32_t shp = kPackZero;
+                if (gs < nseg) {
+                    const int dx = gs % sxn - nx0, t2 = gs / sxn;
+                    const int dy = t2 % syn - nx1, dz = t2 / syn - nx2;
+                    const int a0 = ci0 + dx, a1 = ci1 + dy, a2 = ci2 + dz;
+                    if ((p0 || (a0 >= 1 && a0 <= n0)) && (p1 || (a1 >= 1 && a1 <= n1)) &&
+                        (p2 || (a2 >= 1 && a2 <= n2))) {
+                        int32_t c0, c1, c2, s0x, s0y, s0z;
+                        wrap_and_shift(a0, n0, p0, c0, s0x);
+                        wrap_and_shift(a1, n1, p1, c1, s0y);
+                        wrap_and_shift(a2, n2, p2, c2, s0z);
+                        const int uc = cb + (c0 - 1) + (c1 - 1) * n0 + (c2 - 1) * n0 * n1;
+                        st = L.cs[uc];
+                        len = L.cs[uc + 1] - st;
+                        shp = pack_wind(s0x, s0y, s0z);
+                    }
+                }
+                const int incl = wave_incl_scan(len);
+                const int ex = incl - len;
+                const int total = __builtin_amdgcn_readlane(incl, kWave - 1);
+                const bool ne = len > 0;
+                const int q = (int)mbcnt64(__builtin_amdgcn_ballot_w64(ne));
+                if (ne) {
+                    Rw.dl[q] = st - ex;
+                    Rw.sh[q] = shp;
+                }
+                // home atom hg + a is candidate ex(center) + hg + a of the
+                // center segment, when that segment is in this batch
+                const int cl = center - sb;
+                const int self0 = (cl >= 0 && cl < kWave) ? __builtin_amdgcn_readlane(ex, cl) : -(1 << 28);
+                for (int k0 = 0; k0 < total; k0 += kPassCands) {
+                    const int nch = sgpr(min(kBrickChunks, (total - k0 + kWave - 1) / kWave));
+                    // marks where a non-empty segment other than the first starts
+                    if (lane < kPassCands / 4) Rw.flag[lane] = 0u;
+                    const int pp = ex - k0;
+                    if (ne && q > 0 && pp > 0 && pp < kPassCands) fl[pp] = 1;
+                    int acc = (int)__popcll(__builtin_amdgcn_ballot_w64(ne && q > 0 && pp <= 0));
+                    double cx[kBrickChunks], cy[kBrickChunks], cz[kBrickChunks];
+                    uint64_t ent[kBrickChunks];
+#pragma unroll
+                    for (int g = 0; g < kBrickChunks; ++g) {
+                        const uint64_t bm = __builtin_amdgcn_ballot_w64(fl[g * kWave + lane] != 0);
+                        const int qk = acc + (int)(bm & 1u) + (int)mbcnt64(bm >> 1);
+                        acc += (int)__popcll(bm);
+                        const int kk = k0 + g * kWave + lane;
+                        const int li = kk < total ? kk + Rw.dl[qk] : kUnitAtoms;   // else the far sentinel
+                        const uint32_t sh = Rw.sh[qk];
+                        const LAtom c = L.a[li];
+                        cx[g] = c.x;
+                        cy[g] = c.y;
+                        cz[g] = c.z;
+                        const uint32_t t = sh - c.t + kPackZero;   // pack(s - w_j)
+                        const uint32_t code = (t & 3u) | ((t >> 8) & 12u) | ((t >> 16) & 48u);
+                        const uint32_t lo = narrow ? ((uint32_t)c.j | code << 26) : (uint32_t)c.j;
+                        ent[g] = (uint64_t)lo | ((uint64_t)t << 32);
+                    }
+                    pass_home_atoms<WANT_R, false>(narrow, nch, cx, cy, cz, ent, L.a + hb, nh, ht_l, hj_l,
+                                                   self0 + hg - k0, sd.C, sd.rc2, sd.atom_base, A.slots, A.slotsR,
+                                                   A.cap, A.status, cnt_l);
+                }
+            }
+            if (lane < nh) A.pair_count[sd.atom_base + hj_l] = cnt_l | (narrow ? 0 : kRow8Bit);
+        }
+    }
+}
+
+template <bool COUNT, bool R, typename TI>
+static void small_launch(const PipelineBufs &b, hipStream_t st)
+{
+    SmallArgs A{b.X,          b.sys,     b.sys_atom_base, b.sys_cell_base, b.units,  b.unit_order,
+                b.atom_sys,   b.rec,     b.perm_out,      b.cell_id_out,   b.cell_offsets_out,
+                b.pair_count, b.slots,   b.slotsR,        b.slot_cap,      b.status};
+    hipLaunchKernelGGL((small_kernel<COUNT, R, TI>), dim3((unsigned)b.nunits), dim3(kSmallThreads), 0, st, A);
+}
+
+void launch_small(const PipelineBufs &b, bool count, bool want_R, hipStream_t st)
+{
+    if (b.nunits <= 0) return;
+    if (b.ti_bytes == 8) {
+        if (!count) small_launch<false, false, int64_t>(b, st);
+        else if (want_R) small_launch<true, true, int64_t>(b, st);
+        else small_launch<true, false, int64_t>(b, st);
+    } else {
+        if (!count) small_launch<false, false, int32_t>(b, st);
+        else if (want_R) small_launch<true, true, int32_t>(b, st);
+        else small_launch<true, false, int32_t>(b, st);
     }
 }
 
@@ -1139,8 +1491,14 @@ static void count_launch(const PipelineBufs &b, hipStream_t st)
                            b.pair_count, b.slots, b.slotsR, b.slot_cap, b.status);
 }
 
+void launch_small(const PipelineBufs &b, bool count, bool want_R, hipStream_t st);
+
 void launch_count(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st)
 {
+    if (b.nunits > 0) {   // cell list, counts and slot rows of every unit of small systems
+        launch_small(b, true, want_R, st);
+        return;
+    }
     if (b.nat == 0 || b.nbricks == 0) return;
     if (want_R) {
         if (wide) count_launch<true, true>(b, st);

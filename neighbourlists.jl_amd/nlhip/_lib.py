@@ -69,11 +69,12 @@ class Geometry(C.Structure):
 class Tuning(C.Structure):
     """nlh_tuning: launch configuration (identical results for every value)."""
     _fields_ = [("brick_atoms", C.c_int32), ("fill_block", C.c_int32), ("fill_pairs", C.c_int32),
-                ("count_block", C.c_int32), ("cell_sort", C.c_int32)]
+                ("count_block", C.c_int32), ("cell_sort", C.c_int32), ("fused_small", C.c_int32)]
 
     def __repr__(self):
         return (f"Tuning(brick_atoms={self.brick_atoms}, fill_block={self.fill_block}, "
-                f"fill_pairs={self.fill_pairs}, count_block={self.count_block}, cell_sort={self.cell_sort})")
+                f"fill_pairs={self.fill_pairs}, count_block={self.count_block}, cell_sort={self.cell_sort}, "
+                f"fused_small={self.fused_small})")
 
 
 class PairListView(C.Structure):
@@ -282,8 +283,9 @@ class Handle:
         check(lib().nlh_synchronize(self._h), "synchronize")
 
     def set_tuning(self, brick_atoms: int, fill_block: int, fill_pairs: int = 256, count_block: int = 0,
-                   cell_sort: int = 0):
-        t = Tuning(int(brick_atoms), int(fill_block), int(fill_pairs), int(count_block), int(cell_sort))
+                   cell_sort: int = 0, fused_small: int = 0):
+        t = Tuning(int(brick_atoms), int(fill_block), int(fill_pairs), int(count_block), int(cell_sort),
+                   int(fused_small))
         check(lib().nlh_set_tuning(self._h, C.byref(t)), "set_tuning")
 
     def get_tuning(self) -> "Tuning":

@@ -125,6 +125,8 @@ def test_tuning_rejects_bad_values():
         h.set_tuning(1024, 256, 256, 128)
     with pytest.raises(_lib.NLHInvalidError):
         h.set_tuning(1024, 256, 256, 256, 2)
+    with pytest.raises(_lib.NLHInvalidError):
+        h.set_tuning(1024, 256, 256, 256, 0, 2)
 
 
 def test_autotune_keeps_results():
@@ -178,3 +180,112 @@ def test_replay_after_entries_widen():
         _same(nl, oracle.neighbour_list(Xf, s.cutoff, s.cell, s.pbc, want_R=True))
     assert h.graph_launches() >= 2
     h.close()
+
+
+# ---------------------------------------------------------------------------
+# fused build of small systems
+# ---------------------------------------------------------------------------
+
+def _fused_off(bb):
+    t = bb.handle.get_tuning()
+    bb.handle.set_tuning(t.brick_atoms, t.fill_block, t.fill_pairs, t.count_block, t.cell_sort, 1)
+
+
+def _cell_lists(bb, nat, ncell_entries, ti):
+    perm = np.empty(nat, ti)
+    cid = np.empty(nat, ti)
+    coff = np.empty(ncell_entries, ti)
+    Xs = np.empty((nat, 3))
+    _lib.check(_lib.lib().nlh_copy_celllist(bb.handle.ptr, Xs.ctypes.data, perm.ctypes.data, cid.ctypes.data,
+                                            coff.ctypes.data))
+    return Xs, perm, cid, coff
+
+
+@pytest.mark.parametrize("int_type,want_R", [(nlhip.Int32, False), (nlhip.Int32, True), (nlhip.Int64, True)])
+def test_fused_small_systems_match_the_general_pipeline(int_type, want_R):
+    """A batch of small systems built by the fused kernel (one workgroup per
+    unit of systems: binning, sort, cell list and count in LDS) equals the
+    general pipeline's build of the same batch in every output (pair list,
+    per-system offsets, sorted positions, perm, cell_id, cell_offsets), and
+    the oracle on sampled systems."""
+    systems = workloads.batch_systems(600, seed=181, nmax=2048)
+    X, offs = pack_systems(systems)
+    fused = BatchBuilder(int_type)
+    plain = BatchBuilder(int_type)
+    _fused_off(plain)
+    print(f"fused batch {int_type} R={want_R}", flush=True)
+    a = fused.run(systems, want_R=want_R)
+    b = plain.run(systems, want_R=want_R)
+    assert a.npairs == b.npairs
+    _same(a, b)
+    assert np.array_equal(a.pair_offsets, b.pair_offsets)
+    ti = fused.ti
+    dt = np.int32 if ti == 4 else np.int64
+    M = sum(int(_lib.geometry(s.cell, s.cutoff, s.pbc, ti).ncells_total) for s in systems)
+    ca = _cell_lists(fused, X.shape[0], M + len(systems), dt)
+    cb = _cell_lists(plain, X.shape[0], M + len(systems), dt)
+    for u, v in zip(ca, cb):
+        assert np.array_equal(u, v)
+    _check_oracle(a, systems, X, offs, [0, 1, 299, 599], R=want_R)
+
+
+def test_fused_small_windings_and_fallback():
+    """Atoms outside their cell (windings of one image: 8-byte slot rows) stay
+    on the fused path; a winding beyond +-511 cells makes the build fall back
+    to the general pipeline, with the same results; repeated rebuilds."""
+    systems = workloads.batch_systems(200, seed=182, nmax=1500)
+    moved = []
+    for k, s in enumerate(systems):
+        Xs = s.X.copy()
+        if k % 3 == 0 and s.nat > 1:
+            Xs[::2] += s.cell[0] * (1 if s.pbc[0] else 0) - s.cell[1] * (1 if s.pbc[1] else 0)
+        moved.append(workloads.System(Xs, s.cell, s.pbc, s.cutoff))
+    X1, offs = pack_systems(moved)
+    bb = BatchBuilder()
+    print("fused build with windings", flush=True)
+    res = bb.run(moved, want_R=True)
+    _check_oracle(res, moved, X1, offs, range(0, 200, 7))
+    for rep in range(2):
+        print(f"fused rebuild {rep}", flush=True)
+        g = bb.rebuild(X1, want_R=True)
+        _same(g, res)
+    far = X1.copy()
+    per = [k for k, s in enumerate(systems) if s.pbc[0] and s.nat > 1][:2]
+    assert len(per) == 2
+    a, b = per
+    far[offs[a]:offs[a + 1]] += 1000.0 * systems[a].cell[0]   # every atom 1000 cells along a_1
+    far[offs[b]] += 700.0 * systems[b].cell[0]                # one atom 700 cells along a_1
+    print("fused build with far windings", flush=True)
+    g = bb.rebuild(far, want_R=True)
+    _check_oracle(g, moved, far, offs, [a, b, 7, 100])
+    g2 = bb.rebuild(far, want_R=True)
+    _same(g, g2)
+    plain = BatchBuilder()
+    _fused_off(plain)
+    plain.run(moved, want_R=True)
+    _same(plain.rebuild(far, want_R=True), g)
+
+
+def test_fused_small_tiny_cells_and_empty_systems():
+    """Systems of 1-4 atoms in cells smaller than the cutoff (reach 2-3
+    cells: up to 343 images, several segment batches per home cell), and
+    systems with no atoms, through the fused path."""
+    rng = np.random.default_rng(183)
+    systems = []
+    for k in range(150):
+        if k % 10 == 0:
+            systems.append(workloads.System(np.zeros((0, 3)), np.eye(3) * 9.0, (True, True, True), 4.0))
+            continue
+        side = 2.2 + 2.0 * rng.random()
+        C = np.diag([side, side * (1 + 0.1 * rng.random()), side])
+        n = 1 + k % 4
+        systems.append(workloads.System(rng.random((n, 3)) @ C, C, (bool(k % 2), True, bool(k % 3)),
+                                        4.0 + 2.0 * rng.random()))
+    X, offs = pack_systems(systems)
+    bb = BatchBuilder()
+    print("fused tiny cells", flush=True)
+    res = bb.run(systems, want_R=True)
+    _check_oracle(res, systems, X, offs, range(150))
+    plain = BatchBuilder()
+    _fused_off(plain)
+    _same(plain.run(systems, want_R=True), res)
