@@ -201,7 +201,8 @@ def _batched(nsys: int, steps: int):
     assert Pp == P, (Pp, P)
     return {"nsys": nsys, "atoms": int(offs[-1]), "pairs": int(P), "batched_ms": round(tb * 1e3, 3),
             "tuning": {"brick_atoms": tuned.brick_atoms, "fill_block": tuned.fill_block,
-                       "fill_pairs": tuned.fill_pairs, "count_block": tuned.count_block},
+                       "fill_pairs": tuned.fill_pairs, "count_block": tuned.count_block,
+                       "fused_small": tuned.fused_small},
             "batched_systems_per_s": round(nsys / tb, 1), "plain_ms": round(tp * 1e3, 3),
             "plain_systems_per_s": round(nsys / tp, 1), "speedup": round(tp / tb, 2)}
 

@@ -216,8 +216,9 @@ int nlh_neighbour_list_noalloc(void *stream, const double *X, int64_t nat, const
  * the atoms by cell through buckets of cells counted in LDS wherever the
  * cell grid allows (up to 8192 buckets of 4096 cells), 1 uses one global
  * atomic histogram over the cells.  fused_small: 0 (default) builds a batch
- * of at least 64 systems, each with at most 2048 atoms, 2048 cells and a
- * neighbour-cell reach of at most 3 per axis, by one workgroup per unit of
+ * of at least 64 systems, each with at most 2048 atoms, 2048 cells, 256
+ * cells per axis and a neighbour-cell reach of at most 3 per axis, by one
+ * workgroup per unit of
  * consecutive systems (cell list, counts and slot rows in one launch, all in
  * LDS); 1 always uses the general pipeline.
  * Every configuration gives identical results; no reference counterpart

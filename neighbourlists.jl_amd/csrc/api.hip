@@ -512,7 +512,7 @@ void plan_units(nlh_handle *h)
         total += sd.nat;
         if (sd.nat > nlh::kUnitAtoms || sd.M > nlh::kUnitCells) return;
         for (int k = 0; k < 3; ++k)
-            if (sd.nxyz[k] > nlh::kUnitMaxReach) return;
+            if (sd.nxyz[k] > nlh::kUnitMaxReach || sd.n[k] > nlh::kUnitMaxAxisCells) return;
     }
     if (total == 0) return;   // no count stage follows: the general build writes the cell list
     std::vector<int64_t> work;

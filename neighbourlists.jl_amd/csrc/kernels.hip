@@ -360,25 +360,26 @@ __global__ __launch_bounds__(kScanThreads) void scan_pairs_kernel(
 }
 
 // Per-system `first` arrays for a batch (local, 1-based) and the per-system
-// pair bases.
+// pair bases.  System s owns entries [atom_base[s] + s, atom_base[s+1] + s] of
+// the concatenated first arrays: thread g writes the entry of atom g (its
+// system from atom_sys) and, for g in [1, nsys], the trailing entry of
+// system g - 1 (its pair count + 1).
 template <typename TI>
 __global__ __launch_bounds__(256) void batch_first_kernel(const int64_t *__restrict__ pair_off, int64_t nat,
-                                                          const int64_t *__restrict__ sys_atom_base, int nsys,
+                                                          const int64_t *__restrict__ sys_atom_base,
+                                                          const int32_t *__restrict__ atom_sys, int nsys,
                                                           void *__restrict__ first_out, int64_t *__restrict__ sys_pair_base)
 {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g < nsys + 1) sys_pair_base[g] = pair_off[sys_atom_base[g]];
-    if (g >= nat + nsys) return;
-    // entry g of the concatenated first arrays: system s owns entries
-    // [atom_base[s] + s, atom_base[s+1] + s]
-    int lo = 0, hi = nsys - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (sys_atom_base[mid] + mid <= g) lo = mid;
-        else hi = mid - 1;
+    if (g <= nsys) {
+        const int64_t e = pair_off[sys_atom_base[g]];
+        sys_pair_base[g] = e;
+        if (g > 0) store_ti<TI>(first_out, sys_atom_base[g] + g - 1, e - pair_off[sys_atom_base[g - 1]] + 1);
     }
-    const int64_t a = g - lo;   // global atom index (or one past the system's end)
-    store_ti<TI>(first_out, g, pair_off[a] - pair_off[sys_atom_base[lo]] + 1);
+    if (g < nat) {
+        const int s = atom_sys[g];
+        store_ti<TI>(first_out, g + s, pair_off[g] - pair_off[sys_atom_base[s]] + 1);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -827,9 +828,10 @@ void launch_scan_pairs(const PipelineBufs &b, hipStream_t st)
                                        b.pair_capacity, b.tile_atom, b.tile_cap, fill_tile_shift(b.fill_pairs)));
     if (b.nsys > 1)
         NLH_TI_DISPATCH(b.ti_bytes,
-                        hipLaunchKernelGGL(batch_first_kernel<TI>, dim3(blocks_for(n + b.nsys + 1, 256)), dim3(256),
-                                           0, st, b.pair_off, n, b.sys_atom_base, b.nsys, b.first_out,
-                                           b.sys_pair_base));
+                        hipLaunchKernelGGL(batch_first_kernel<TI>,
+                                           dim3(blocks_for(std::max<int64_t>(n, (int64_t)b.nsys + 1), 256)),
+                                           dim3(256), 0, st, b.pair_off, n, b.sys_atom_base, b.atom_sys, b.nsys,
+                                           b.first_out, b.sys_pair_base));
 }
 
 void launch_sorted_positions(const PipelineBufs &b, double *Xs, hipStream_t st)

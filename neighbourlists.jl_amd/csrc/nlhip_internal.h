@@ -292,6 +292,7 @@ constexpr int kUnitAtoms = 2048;      // atoms of one unit (and of one system) a
 constexpr int kUnitCells = 2048;      // cells of one unit at most
 constexpr int kUnitSystems = 256;     // systems of one unit at most
 constexpr int kUnitMaxReach = 3;      // neighbour-cell reach per axis at most
+constexpr int kUnitMaxAxisCells = 256;   // cells per axis of one system at most
 void launch_small(const PipelineBufs &b, bool count, bool want_R, hipStream_t st);
 
 }  // namespace nlh

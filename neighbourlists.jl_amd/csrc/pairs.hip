@@ -985,6 +985,7 @@ struct SmallSort {                 // LDS of the sort
     int16_t grp[kUnitAtoms];       // atoms grouped by cell, in arrival order
     int32_t fill[kUnitCells];      // per cell: atoms, then arrivals
     uint32_t wk[kUnitAtoms];       // packed winding of each atom
+    uint8_t sy[kUnitAtoms];        // unit system of each atom
 };
 struct SmallRows {                 // LDS of one wave in the count
     int32_t dl[kWave];             // LDS offset less candidate prefix of each non-empty segment
@@ -997,6 +998,8 @@ struct SmallLDS {
     int32_t sys_cell0[kUnitSystems + 1];
     int32_t sys_atom0[kUnitSystems + 1];
     int32_t sys_row8[kUnitSystems];   // 8-byte slot rows: some winding, or reach beyond one cell
+    int32_t next_cell;                // the next home cell a wave of the count takes
+    uint32_t cell_info[kUnitCells];   // per unit cell: system | (c1 - 1) << 8 | (c2 - 1) << 16 | (c3 - 1) << 24
     union {
         SmallSort s;
         SmallRows r[kSmallThreads / kWave];
@@ -1024,6 +1027,30 @@ struct SmallArgs {
     Status *__restrict__ status;
 };
 
+// wrap_and_shift (cell_list.jl:105-120) of a neighbour cell index a within a
+// few cells of [1, n] on a periodic axis (rn = 1 / n as float: the floor
+// of (a - 1/2) / n is exact for |a| < 1024); on an open axis only in-bounds
+// indices reach here.
+__device__ __forceinline__ void wrap_small(int32_t a, int32_t n, float rn, bool p, int32_t &w, int32_t &s)
+{
+    if (!p) {
+        w = a;
+        s = 0;
+        return;
+    }
+    const int32_t q = (int32_t)floorf(((float)a - 0.5f) * rn);
+    w = a - q * n;
+    s = q;
+}
+
+// The next work item of a wave from a workgroup counter (wave-uniform).
+__device__ __forceinline__ int take_next(int32_t *ctr)
+{
+    int t = 0;
+    if (lane_id() == 0) t = atomicAdd(ctr, 1);
+    return __builtin_amdgcn_readfirstlane(t);
+}
+
 template <bool COUNT, bool WANT_R, typename TI>
 __global__ __launch_bounds__(kSmallThreads) void small_kernel(SmallArgs A)
 {
@@ -1050,6 +1077,7 @@ __global__ __launch_bounds__(kSmallThreads) void small_kernel(SmallArgs A)
         f.j = 0;
         f.t = kPackZero;
         L.a[kUnitAtoms] = f;
+        L.next_cell = kSmallThreads / kWave;   // wave w starts at home cell w
     }
     __syncthreads();
     // cell and winding of every atom (position_to_cell_index + bin_wrap_and_shift,
@@ -1065,6 +1093,7 @@ __global__ __launch_bounds__(kSmallThreads) void small_kernel(SmallArgs A)
         const uint32_t wp = pack_wind(w[0], w[1], w[2]);
         L.u.s.key[i] = (int16_t)key;
         L.u.s.wk[i] = wp;
+        L.u.s.sy[i] = (uint8_t)k;
         if (wp != kPackZero) L.sys_row8[k] = 1;
         big |= (wp & kWindBig) != 0;
         atomicAdd(&L.u.s.fill[key], 1);
@@ -1098,7 +1127,7 @@ __global__ __launch_bounds__(kSmallThreads) void small_kernel(SmallArgs A)
         int rank = 0;
         for (int t = st; t < en; ++t) rank += L.u.s.grp[t] < i;
         const int p = st + rank;
-        const int k = find_sys(L.sys_atom0, ns, (int32_t)i);
+        const int k = L.u.s.sy[i];
         const int64_t g = A0 + i;
         LAtom la;
         la.x = A.X[3 * g];
@@ -1117,9 +1146,13 @@ __global__ __launch_bounds__(kSmallThreads) void small_kernel(SmallArgs A)
         store_ti<TI>(A.perm_out, A0 + p, (int64_t)la.j + 1);
         store_ti<TI>(A.cell_id_out, A0 + p, (int64_t)(key - L.sys_cell0[k]) + 1);
     }
-    // cell offsets: per system, 1-based, its cells then a trailing entry
+    // cell offsets: per system, 1-based, its cells then a trailing entry;
+    // the system and the grid coordinates of every cell for the count
     for (int c = tid; c < M_u; c += kSmallThreads) {
         const int k = find_sys(L.sys_cell0, ns, (int32_t)c);
+        const int lc = c - L.sys_cell0[k], n0 = A.sys[s0 + k].n[0], n1 = A.sys[s0 + k].n[1];
+        L.cell_info[c] = (uint32_t)k | (uint32_t)(lc % n0) << 8 | (uint32_t)((lc / n0) % n1) << 16 |
+                         (uint32_t)(lc / (n0 * n1)) << 24;
         const int64_t gc = (int64_t)C0 + c + s0 + k;
         store_ti<TI>(A.cell_offsets_out, gc, (int64_t)(L.cs[c] - L.sys_atom0[k]) + 1);
         if (c == L.sys_cell0[k + 1] - 1)
@@ -1129,17 +1162,22 @@ __global__ __launch_bounds__(kSmallThreads) void small_kernel(SmallArgs A)
     __syncthreads();   // the sorted atoms are complete; the sort's LDS now holds the waves' segment maps
     SmallRows &Rw = L.u.r[wv];
     uint8_t *fl = reinterpret_cast<uint8_t *>(Rw.flag);
-    for (int hc = wv; hc < M_u; hc += kSmallThreads / kWave) {
+    for (int hc = wv; hc < M_u; hc = take_next(&L.next_cell)) {
         const int h0 = sgpr(L.cs[hc]), nha = sgpr(L.cs[hc + 1]) - h0;
         if (nha == 0) continue;
-        const int k = sgpr(find_sys(L.sys_cell0, ns, (int32_t)hc));
+        const uint32_t info = (uint32_t)sgpr((int)L.cell_info[hc]);
+        const int k = (int)(info & 255u);
         const SysDev &sd = A.sys[s0 + k];
         const int cb = sgpr(L.sys_cell0[k]);
         const int n0 = sd.n[0], n1 = sd.n[1], n2 = sd.n[2];
-        const int lc = hc - cb;
-        const int ci0 = lc % n0 + 1, ci1 = (lc / n0) % n1 + 1, ci2 = lc / (n0 * n1) + 1;
+        const int ci0 = (int)((info >> 8) & 255u) + 1, ci1 = (int)((info >> 16) & 255u) + 1, ci2 = (int)(info >> 24) + 1;
         const int nx0 = sd.nxyz[0], nx1 = sd.nxyz[1], nx2 = sd.nxyz[2];
         const int sxn = 2 * nx0 + 1, syn = 2 * nx1 + 1;
+        // quotients of the small integers below by float reciprocals: exact,
+        // as every numerator is within a few hundred of zero and a half away
+        // from a multiple of the divisor
+        const float rsx = 1.0f / (float)sxn, rsy = 1.0f / (float)syn;
+        const float rn0 = 1.0f / (float)n0, rn1 = 1.0f / (float)n1, rn2 = 1.0f / (float)n2;
         const int nseg = sxn * syn * (2 * nx2 + 1);
         const int center = (nx2 * syn + nx1) * sxn + nx0;   // (dx, dy, dz) = 0: the home cell, zero shift
         const bool p0 = (sd.pbc & 1) != 0, p1 = (sd.pbc & 2) != 0, p2 = (sd.pbc & 4) != 0;
@@ -1159,15 +1197,15 @@ __global__ __launch_bounds__(kSmallThreads) void small_kernel(SmallArgs A)
                 int st = 0, len = 0;
                 uint32_t shp = kPackZero;
                 if (gs < nseg) {
-                    const int dx = gs % sxn - nx0, t2 = gs / sxn;
-                    const int dy = t2 % syn - nx1, dz = t2 / syn - nx2;
+                    const int t2 = (int)(((float)gs + 0.5f) * rsx), t3 = (int)(((float)t2 + 0.5f) * rsy);
+                    const int dx = gs - t2 * sxn - nx0, dy = t2 - t3 * syn - nx1, dz = t3 - nx2;
                     const int a0 = ci0 + dx, a1 = ci1 + dy, a2 = ci2 + dz;
                     if ((p0 || (a0 >= 1 && a0 <= n0)) && (p1 || (a1 >= 1 && a1 <= n1)) &&
                         (p2 || (a2 >= 1 && a2 <= n2))) {
                         int32_t c0, c1, c2, s0x, s0y, s0z;
-                        wrap_and_shift(a0, n0, p0, c0, s0x);
-                        wrap_and_shift(a1, n1, p1, c1, s0y);
-                        wrap_and_shift(a2, n2, p2, c2, s0z);
+                        wrap_small(a0, n0, rn0, p0, c0, s0x);
+                        wrap_small(a1, n1, rn1, p1, c1, s0y);
+                        wrap_small(a2, n2, rn2, p2, c2, s0z);
                         const int uc = cb + (c0 - 1) + (c1 - 1) * n0 + (c2 - 1) * n0 * n1;
                         st = L.cs[uc];
                         len = L.cs[uc + 1] - st;
