@@ -140,9 +140,10 @@ struct nlh_handle {
     int64_t *atom_base_d = nullptr;
     int32_t *cell_base_d = nullptr, *brick_base_d = nullptr;
     // fused build of small systems: units of consecutive systems (first
-    // system of each, then the end) and their launch order, in the tables
+    // system of each, then the end) and the work items in launch order
+    // (unit, then part | parts << 16), in the tables
     std::vector<int32_t> units_host, order_host;
-    int32_t nunits = 0;
+    int32_t nunits = 0, nitems = 0;
     int32_t *units_d = nullptr, *order_d = nullptr;
     bool small_ok = true;   // cleared when a fused build met a winding beyond +-511
     std::vector<char> tables_host;        // bytes of the last upload
@@ -255,6 +256,7 @@ struct nlh_handle {
         b.fill_pairs = tune.fill_pairs;
         b.count_block = count_block();
         b.nunits = nunits;
+        b.nitems = nitems;
         b.units = units_d;
         b.unit_order = order_d;
         return b;
@@ -495,16 +497,31 @@ void assign_bricks(nlh_handle *h)
 // Fused build of small systems: a batch of at least kSmallMinSystems systems,
 // each within a unit's limits (atoms, cells, neighbour-cell reach), is cut
 // into units of consecutive systems, at most kUnitAtoms atoms, kUnitCells
-// cells and kUnitSystems systems each, launched largest first.  Otherwise
-// (or when turned off, or after a winding beyond +-511 made a fused build
-// fall back) no units: the general pipeline.
+// cells and kUnitSystems systems each.  A unit whose estimated count work is
+// well above an even share of the batch is split into parts (each sorts the
+// whole unit and counts every parts-th home cell), and the work items are
+// launched largest first.  Otherwise (or when turned off, or after a winding
+// beyond +-511 made a fused build fall back) no units: the general pipeline.
 constexpr int kSmallMinSystems = 64;
+constexpr int kMaxUnitParts = 16;       // parts of one unit at most (each at least 4 home cells)
+constexpr double kItemRounds = 3.0;     // work items per CU the split aims at
+
+// Estimated count work of a system in distance tests of 64 candidates: every
+// atom against the atoms of its (2 nxyz + 1)^3 neighbour cells, the cells
+// holding the system's mean number of atoms.
+double small_count_work(const nlh::SysDev &sd)
+{
+    if (sd.nat <= 0) return 0.0;
+    const double nseg = (2.0 * sd.nxyz[0] + 1) * (2.0 * sd.nxyz[1] + 1) * (2.0 * sd.nxyz[2] + 1);
+    return (double)sd.nat * std::ceil(nseg * sd.nat / std::max(sd.M, 1) / 64.0);
+}
 
 void plan_units(nlh_handle *h)
 {
     h->units_host.clear();
     h->order_host.clear();
     h->nunits = 0;
+    h->nitems = 0;
     const int nsys = (int)h->sys_host.size();
     if (!h->small_ok || h->wide || h->tune.fused_small != 0 || nsys < kSmallMinSystems) return;
     int64_t total = 0;
@@ -515,25 +532,56 @@ void plan_units(nlh_handle *h)
             if (sd.nxyz[k] > nlh::kUnitMaxReach || sd.n[k] > nlh::kUnitMaxAxisCells) return;
     }
     if (total == 0) return;   // no count stage follows: the general build writes the cell list
-    std::vector<int64_t> work;
+    std::vector<double> work;
+    std::vector<int64_t> atoms_u, cells_u;
     int s = 0;
     while (s < nsys) {
         const int first = s;
         int64_t atoms = 0, cells = 0;
+        double w = 0.0;
         while (s < nsys && s - first < nlh::kUnitSystems && atoms + h->sys_host[s].nat <= nlh::kUnitAtoms &&
                cells + h->sys_host[s].M <= nlh::kUnitCells) {
             atoms += h->sys_host[s].nat;
             cells += h->sys_host[s].M;
+            w += small_count_work(h->sys_host[s]);
             ++s;
         }
         h->units_host.push_back(first);
-        work.push_back(atoms);
+        work.push_back(w);
+        atoms_u.push_back(atoms);
+        cells_u.push_back(cells);
     }
     h->units_host.push_back(nsys);
     h->nunits = (int32_t)work.size();
-    h->order_host.resize(h->nunits);
-    for (int u = 0; u < h->nunits; ++u) h->order_host[u] = u;
-    std::stable_sort(h->order_host.begin(), h->order_host.end(), [&](int a, int b) { return work[a] > work[b]; });
+    // A workgroup's time, fitted on MI355X (us): 14.4 + 0.0115 per atom of
+    // the unit (its sort, done by every part) + 0.0072 per estimated test +
+    // 0.154 per cell (the last two shared by the parts).
+    std::vector<double> fixed(h->nunits), shared(h->nunits);
+    double time_total = 0.0;
+    for (int u = 0; u < h->nunits; ++u) {
+        fixed[u] = 14.4 + 0.0115 * atoms_u[u];
+        shared[u] = 0.0072 * work[u] + 0.154 * cells_u[u];
+        time_total += fixed[u] + shared[u];
+    }
+    const double share = std::max(1.0, time_total / (std::max(h->cus, 1) * kItemRounds));
+    struct Item {
+        double key;
+        int32_t u, part, parts;
+    };
+    std::vector<Item> items;
+    for (int u = 0; u < h->nunits; ++u) {
+        const int cap = (int)std::max<int64_t>(1, std::min<int64_t>(kMaxUnitParts, cells_u[u] / 4));
+        const int parts = (int)std::max<double>(1.0, std::min<double>(cap, std::llround(shared[u] / share)));
+        const double key = fixed[u] + shared[u] / parts;
+        for (int p = 0; p < parts; ++p) items.push_back({key, u, p, parts});
+    }
+    std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) { return a.key > b.key; });
+    h->nitems = (int32_t)items.size();
+    h->order_host.resize(2 * items.size());
+    for (size_t t = 0; t < items.size(); ++t) {
+        h->order_host[2 * t] = items[t].u;
+        h->order_host[2 * t + 1] = items[t].part | items[t].parts << 16;
+    }
 }
 
 // Upload the system tables (sys, atom/cell/brick bases) as one copy through

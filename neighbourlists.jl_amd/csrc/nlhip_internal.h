@@ -265,8 +265,9 @@ struct PipelineBufs {
     // fused build of small systems (units of consecutive systems, one
     // workgroup each): 0 units = the general pipeline
     int32_t nunits = 0;
+    int32_t nitems = 0;                   // work items: parts of units, one workgroup each
     const int32_t *units = nullptr;       // nunits + 1: first system of each unit
-    const int32_t *unit_order = nullptr;  // nunits: launch order (largest unit first)
+    const int32_t *unit_order = nullptr;  // 2 nitems: unit, part | parts << 16 (launch order, largest first)
     int fill_block;          // threads per fill workgroup (multiple of 64)
     int fill_pairs;          // output pairs per fill wave (128, 256 or 512)
     int count_block;         // threads per count-pass brick workgroup (256 or 512)

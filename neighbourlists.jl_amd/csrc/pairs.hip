@@ -970,15 +970,27 @@ __global__ __launch_bounds__(BT) void count_kernel(
 // Fused build of small systems
 // ---------------------------------------------------------------------------
 //
-// One workgroup of kSmallThreads threads per unit of consecutive systems (at
-// most kUnitAtoms atoms, kUnitCells cells and kUnitSystems systems), all of
-// it in LDS: the cell and winding of every atom, a stable counting sort by
-// cell (ties in ascending original index, Julia's stable sortperm), the
-// cell-list outputs, and -- with COUNT -- every atom's neighbour count and
-// slot row, through the brick pass's tests and emission.  A home cell's
+// One workgroup of kSmallThreads threads per work item: a unit of
+// consecutive systems (at most kUnitAtoms atoms, kUnitCells cells and
+// kUnitSystems systems) or one of its parts, all of it in LDS: the cell and
+// winding of every atom, a stable counting sort by cell (ties in ascending
+// original index, Julia's stable sortperm), the cell-list outputs (written
+// by part 0), and -- with COUNT -- the neighbour count and slot row of every
+// atom in the part's home cells (unit cells part, part + parts, ...), through
+// the brick pass's tests and emission; the waves take those home cells from
+// a workgroup counter.  A home cell's
 // candidates are its (2 nxyz + 1)^3 neighbour cells in the reference order
 // (dz, dy, dx outer to inner, gpu_kernels.jl:69-71), each a segment of the
 // sorted atoms with its cell shift, taken 64 segments at a time.
+
+// Count work items: a group of at most sys_group home atoms of one cell.
+// The group size of a system aims at kItemTests distance tests (chunks of 64
+// candidates) per item, at least kMinGroup atoms: a cell whose candidates run
+// to thousands (a small periodic cell, its own 26 images) is shared by
+// several waves instead of holding one wave for all of it.
+constexpr int kItemTests = 512;
+constexpr int kMinGroup = 4;
+constexpr int kUnitItems = kUnitAtoms / kMinGroup + kUnitCells;   // items of one unit at most
 
 struct SmallSort {                 // LDS of the sort
     int16_t key[kUnitAtoms];       // unit cell of each atom
@@ -998,8 +1010,11 @@ struct SmallLDS {
     int32_t sys_cell0[kUnitSystems + 1];
     int32_t sys_atom0[kUnitSystems + 1];
     int32_t sys_row8[kUnitSystems];   // 8-byte slot rows: some winding, or reach beyond one cell
-    int32_t next_cell;                // the next home cell a wave of the count takes
+    int32_t next_item;                // the next count work item a wave takes
+    int32_t sys_group[kUnitSystems];  // home atoms per count work item of each system
     uint32_t cell_info[kUnitCells];   // per unit cell: system | (c1 - 1) << 8 | (c2 - 1) << 16 | (c3 - 1) << 24
+    int16_t item0[kUnitCells + 1];    // first count work item of each unit cell
+    int16_t item_cell[kUnitItems];    // unit cell of each count work item
     union {
         SmallSort s;
         SmallRows r[kSmallThreads / kWave];
@@ -1056,7 +1071,9 @@ __global__ __launch_bounds__(kSmallThreads) void small_kernel(SmallArgs A)
 {
     __shared__ SmallLDS L;
     const int tid = (int)threadIdx.x, lane = lane_id(), wv = sgpr(tid >> 6);
-    const int u = A.order[blockIdx.x];
+    const int u = A.order[2 * blockIdx.x], pinfo = A.order[2 * blockIdx.x + 1];
+    const int part = pinfo & 0xffff, parts = pinfo >> 16;   // this workgroup counts home cells part + parts k
+    const bool writer = part == 0;                           // the first part writes the cell list
     const int s0 = A.units[u], ns = A.units[u + 1] - s0;
     const int64_t A0 = A.sys_atom_base[s0];
     const int nat_u = (int)(A.sys_atom_base[s0 + ns] - A0);
@@ -1068,6 +1085,10 @@ __global__ __launch_bounds__(kSmallThreads) void small_kernel(SmallArgs A)
         if (k < ns) {
             const SysDev &sd = A.sys[s0 + k];
             L.sys_row8[k] = (sd.nxyz[0] > 1 || sd.nxyz[1] > 1 || sd.nxyz[2] > 1) ? 1 : 0;
+            // candidates per home atom: the system's mean cell occupancy times its neighbour cells
+            const int nseg = (2 * sd.nxyz[0] + 1) * (2 * sd.nxyz[1] + 1) * (2 * sd.nxyz[2] + 1);
+            const int chunks = max(1, (int)((nseg * (int64_t)sd.nat / max(sd.M, 1) + kWave - 1) / kWave));
+            L.sys_group[k] = min(kWave, max(kMinGroup, kItemTests / chunks));
         }
     }
     for (int c = tid; c < M_u; c += kSmallThreads) L.u.s.fill[c] = 0;
@@ -1077,7 +1098,7 @@ __global__ __launch_bounds__(kSmallThreads) void small_kernel(SmallArgs A)
         f.j = 0;
         f.t = kPackZero;
         L.a[kUnitAtoms] = f;
-        L.next_cell = kSmallThreads / kWave;   // wave w starts at home cell w
+        L.next_item = kSmallThreads / kWave;   // wave w starts at the part's item w
     }
     __syncthreads();
     // cell and winding of every atom (position_to_cell_index + bin_wrap_and_shift,
@@ -1097,7 +1118,7 @@ __global__ __launch_bounds__(kSmallThreads) void small_kernel(SmallArgs A)
         if (wp != kPackZero) L.sys_row8[k] = 1;
         big |= (wp & kWindBig) != 0;
         atomicAdd(&L.u.s.fill[key], 1);
-        A.atom_sys[g] = s0 + k;   // the fill kernel's map of atoms to systems
+        if (writer) A.atom_sys[g] = s0 + k;   // the fill kernel's map of atoms to systems
     }
     if (bad) atomicOr(&A.status->bad_position, 1);
     if (big) atomicOr(&A.status->small_fallback, 1);   // the general pipeline redoes the build
@@ -1136,15 +1157,17 @@ __global__ __launch_bounds__(kSmallThreads) void small_kernel(SmallArgs A)
         la.j = i - L.sys_atom0[k];
         la.t = L.u.s.wk[i];
         L.a[p] = la;
-        Rec r;
-        r.x = la.x;
-        r.y = la.y;
-        r.z = la.z;
-        r.j = la.j;
-        r.w = la.t;
-        A.rec[A0 + p] = r;
-        store_ti<TI>(A.perm_out, A0 + p, (int64_t)la.j + 1);
-        store_ti<TI>(A.cell_id_out, A0 + p, (int64_t)(key - L.sys_cell0[k]) + 1);
+        if (writer) {
+            Rec r;
+            r.x = la.x;
+            r.y = la.y;
+            r.z = la.z;
+            r.j = la.j;
+            r.w = la.t;
+            A.rec[A0 + p] = r;
+            store_ti<TI>(A.perm_out, A0 + p, (int64_t)la.j + 1);
+            store_ti<TI>(A.cell_id_out, A0 + p, (int64_t)(key - L.sys_cell0[k]) + 1);
+        }
     }
     // cell offsets: per system, 1-based, its cells then a trailing entry;
     // the system and the grid coordinates of every cell for the count
@@ -1154,19 +1177,46 @@ __global__ __launch_bounds__(kSmallThreads) void small_kernel(SmallArgs A)
         L.cell_info[c] = (uint32_t)k | (uint32_t)(lc % n0) << 8 | (uint32_t)((lc / n0) % n1) << 16 |
                          (uint32_t)(lc / (n0 * n1)) << 24;
         const int64_t gc = (int64_t)C0 + c + s0 + k;
-        store_ti<TI>(A.cell_offsets_out, gc, (int64_t)(L.cs[c] - L.sys_atom0[k]) + 1);
-        if (c == L.sys_cell0[k + 1] - 1)
-            store_ti<TI>(A.cell_offsets_out, gc + 1, (int64_t)(L.sys_atom0[k + 1] - L.sys_atom0[k]) + 1);
+        if (writer) {
+            store_ti<TI>(A.cell_offsets_out, gc, (int64_t)(L.cs[c] - L.sys_atom0[k]) + 1);
+            if (c == L.sys_cell0[k + 1] - 1)
+                store_ti<TI>(A.cell_offsets_out, gc + 1, (int64_t)(L.sys_atom0[k + 1] - L.sys_atom0[k]) + 1);
+        }
     }
     if (!COUNT) return;
-    __syncthreads();   // the sorted atoms are complete; the sort's LDS now holds the waves' segment maps
+    __syncthreads();   // cell starts, cell info and the sorted atoms are complete
+    {   // count work items, numbered in cell order: the cells' item counts, scanned
+        const int c = 2 * tid;
+        int v0 = 0, v1 = 0;
+        if (c < M_u) {
+            const int g = L.sys_group[L.cell_info[c] & 255u];
+            v0 = (L.cs[c + 1] - L.cs[c] + g - 1) / g;
+        }
+        if (c + 1 < M_u) {
+            const int g = L.sys_group[L.cell_info[c + 1] & 255u];
+            v1 = (L.cs[c + 2] - L.cs[c + 1] + g - 1) / g;
+        }
+        int64_t tot;
+        const int ex = (int)block_excl_scan_n<kSmallThreads>(v0 + v1, tot);
+        if (c < M_u) L.item0[c] = (int16_t)ex;
+        if (c + 1 < M_u) L.item0[c + 1] = (int16_t)(ex + v0);
+        if (tid == 0) L.item0[M_u] = (int16_t)tot;
+        __syncthreads();
+        for (int t = 0; t < v0; ++t) L.item_cell[ex + t] = (int16_t)c;
+        for (int t = 0; t < v1; ++t) L.item_cell[ex + v0 + t] = (int16_t)(c + 1);
+    }
+    __syncthreads();   // the items are numbered; the sort's LDS now holds the waves' segment maps
+    const int nitems_u = L.item0[M_u];
     SmallRows &Rw = L.u.r[wv];
     uint8_t *fl = reinterpret_cast<uint8_t *>(Rw.flag);
-    for (int hc = wv; hc < M_u; hc = take_next(&L.next_cell)) {
-        const int h0 = sgpr(L.cs[hc]), nha = sgpr(L.cs[hc + 1]) - h0;
-        if (nha == 0) continue;
+    for (int it = part + parts * wv; it < nitems_u; it = part + parts * take_next(&L.next_item)) {
+        const int hc = sgpr(L.item_cell[it]);
+        const int ig = it - sgpr(L.item0[hc]);   // the item's group of home atoms in its cell
+        const int h0 = sgpr(L.cs[hc]), nha_c = sgpr(L.cs[hc + 1]) - h0;
         const uint32_t info = (uint32_t)sgpr((int)L.cell_info[hc]);
         const int k = (int)(info & 255u);
+        const int grp = sgpr(L.sys_group[k]);
+        const int hg0 = ig * grp, nha = min(nha_c, hg0 + grp);   // home atoms [hg0, nha) of the cell
         const SysDev &sd = A.sys[s0 + k];
         const int cb = sgpr(L.sys_cell0[k]);
         const int n0 = sd.n[0], n1 = sd.n[1], n2 = sd.n[2];
@@ -1182,7 +1232,7 @@ __global__ __launch_bounds__(kSmallThreads) void small_kernel(SmallArgs A)
         const int center = (nx2 * syn + nx1) * sxn + nx0;   // (dx, dy, dz) = 0: the home cell, zero shift
         const bool p0 = (sd.pbc & 1) != 0, p1 = (sd.pbc & 2) != 0, p2 = (sd.pbc & 4) != 0;
         const bool narrow = sgpr(L.sys_row8[k]) == 0;   // every S in [-1, 1]^3: 4-byte slot entries
-        for (int hg = 0; hg < nha; hg += kWave) {
+        for (int hg = hg0; hg < nha; hg += kWave) {
             const int nh = sgpr(min(kWave, nha - hg));
             const int hb = h0 + hg;
             const int hl = hb + (lane < nh ? lane : 0);
@@ -1267,7 +1317,7 @@ static void small_launch(const PipelineBufs &b, hipStream_t st)
     SmallArgs A{b.X,          b.sys,     b.sys_atom_base, b.sys_cell_base, b.units,  b.unit_order,
                 b.atom_sys,   b.rec,     b.perm_out,      b.cell_id_out,   b.cell_offsets_out,
                 b.pair_count, b.slots,   b.slotsR,        b.slot_cap,      b.status};
-    hipLaunchKernelGGL((small_kernel<COUNT, R, TI>), dim3((unsigned)b.nunits), dim3(kSmallThreads), 0, st, A);
+    hipLaunchKernelGGL((small_kernel<COUNT, R, TI>), dim3((unsigned)b.nitems), dim3(kSmallThreads), 0, st, A);
 }
 
 void launch_small(const PipelineBufs &b, bool count, bool want_R, hipStream_t st)
