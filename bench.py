@@ -207,6 +207,28 @@ def _batched(nsys: int, steps: int):
             "plain_systems_per_s": round(nsys / tp, 1), "speedup": round(tp / tb, 2)}
 
 
+def _batched_rebuild(nsys: int, steps: int, warmup: int) -> dict:
+    """The batched rebuild alone, default launch configuration (counter runs:
+    no plain path, no autotuner trials)."""
+    from nlhip import _lib, workloads
+    from nlhip.batch import BatchBuilder, pack_systems
+
+    systems = workloads.batch_systems(nsys, seed=7)
+    X, offs = pack_systems(systems)
+    Xd = _lib.DeviceArray(X)
+    bb = BatchBuilder()
+    bb.run(systems, X_device=Xd, copy=False)
+    for _ in range(warmup):
+        bb.rebuild(X, X_device=Xd, copy=False)
+    _sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        P = bb.rebuild(X, X_device=Xd, copy=False)
+    _sync()
+    tb = (time.perf_counter() - t0) / max(steps, 1)
+    return {"nsys": nsys, "atoms": int(offs[-1]), "pairs": int(P), "batched_ms": round(tb * 1e3, 3)}
+
+
 def _served(nreq: int, seed: int = 11) -> dict:
     """Request stream (Poisson arrivals; 16..200,000 atoms; 30% rebuilds of a
     resident system; 64 GB budget) through the continuous-batching server and
@@ -333,6 +355,8 @@ def _only(name: str, steps: int, warmup: int) -> dict:
         return _single(workloads.triclinic_fcc_slab(), steps, warmup)
     if name == "batched":
         return _batched(4096, steps)
+    if name == "batched_rebuild":
+        return _batched_rebuild(4096, steps, warmup)
     if name == "served":
         return _served(steps)
     if name == "copy":
@@ -346,7 +370,8 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--quick", action="store_true", help="headline only")
-    ap.add_argument("--only", choices=["cube_100k", "cube_1M", "fcc_slab_1M", "batched", "served", "copy"],
+    ap.add_argument("--only", choices=["cube_100k", "cube_1M", "fcc_slab_1M", "batched", "batched_rebuild", "served",
+                                       "copy"],
                     help="run one workload alone (profiler runs); prints its own JSON")
     args = ap.parse_args()
     if args.gpus != 1:
