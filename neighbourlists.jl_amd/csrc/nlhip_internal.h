@@ -286,14 +286,15 @@ bool launch_fill(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st);
 void launch_sorted_positions(const PipelineBufs &b, double *Xs, hipStream_t st);
 
 // Fused build of small systems (b.nunits > 0): the cell list of every
-// system, and with count = true also its pair counts and slot rows, by one
-// workgroup per unit of systems.
+// system, its pair counts and its slot rows, by one workgroup per work item
+// (a unit of consecutive systems or one part of one); launched by
+// launch_count, while launch_build launches nothing for such a batch.
 constexpr int kSmallThreads = 1024;
 constexpr int kUnitAtoms = 2048;      // atoms of one unit (and of one system) at most
 constexpr int kUnitCells = 2048;      // cells of one unit at most
 constexpr int kUnitSystems = 256;     // systems of one unit at most
 constexpr int kUnitMaxReach = 3;      // neighbour-cell reach per axis at most
 constexpr int kUnitMaxAxisCells = 256;   // cells per axis of one system at most
-void launch_small(const PipelineBufs &b, bool count, bool want_R, hipStream_t st);
+void launch_small(const PipelineBufs &b, bool want_R, hipStream_t st);
 
 }  // namespace nlh

@@ -975,7 +975,7 @@ __global__ __launch_bounds__(BT) void count_kernel(
 // kUnitSystems systems) or one of its parts, all of it in LDS: the cell and
 // winding of every atom, a stable counting sort by cell (ties in ascending
 // original index, Julia's stable sortperm), the cell-list outputs (written
-// by part 0), and -- with COUNT -- the neighbour count and slot row of every
+// by part 0), and the neighbour count and slot row of every
 // atom in the part's home cells (unit cells part, part + parts, ...), through
 // the brick pass's tests and emission; the waves take those home cells from
 // a workgroup counter.  A home cell's
@@ -1066,7 +1066,7 @@ __device__ __forceinline__ int take_next(int32_t *ctr)
     return __builtin_amdgcn_readfirstlane(t);
 }
 
-template <bool COUNT, bool WANT_R, typename TI>
+template <bool WANT_R, typename TI>
 __global__ __launch_bounds__(kSmallThreads) void small_kernel(SmallArgs A)
 {
     __shared__ SmallLDS L;
@@ -1183,7 +1183,6 @@ __global__ __launch_bounds__(kSmallThreads) void small_kernel(SmallArgs A)
                 store_ti<TI>(A.cell_offsets_out, gc + 1, (int64_t)(L.sys_atom0[k + 1] - L.sys_atom0[k]) + 1);
         }
     }
-    if (!COUNT) return;
     __syncthreads();   // cell starts, cell info and the sorted atoms are complete
     {   // count work items, numbered in cell order: the cells' item counts, scanned
         const int c = 2 * tid;
@@ -1311,26 +1310,24 @@ __global__ __launch_bounds__(kSmallThreads) void small_kernel(SmallArgs A)
     }
 }
 
-template <bool COUNT, bool R, typename TI>
+template <bool R, typename TI>
 static void small_launch(const PipelineBufs &b, hipStream_t st)
 {
     SmallArgs A{b.X,          b.sys,     b.sys_atom_base, b.sys_cell_base, b.units,  b.unit_order,
                 b.atom_sys,   b.rec,     b.perm_out,      b.cell_id_out,   b.cell_offsets_out,
                 b.pair_count, b.slots,   b.slotsR,        b.slot_cap,      b.status};
-    hipLaunchKernelGGL((small_kernel<COUNT, R, TI>), dim3((unsigned)b.nitems), dim3(kSmallThreads), 0, st, A);
+    hipLaunchKernelGGL((small_kernel<R, TI>), dim3((unsigned)b.nitems), dim3(kSmallThreads), 0, st, A);
 }
 
-void launch_small(const PipelineBufs &b, bool count, bool want_R, hipStream_t st)
+void launch_small(const PipelineBufs &b, bool want_R, hipStream_t st)
 {
-    if (b.nunits <= 0) return;
+    if (b.nunits <= 0 || b.nitems <= 0) return;
     if (b.ti_bytes == 8) {
-        if (!count) small_launch<false, false, int64_t>(b, st);
-        else if (want_R) small_launch<true, true, int64_t>(b, st);
-        else small_launch<true, false, int64_t>(b, st);
+        if (want_R) small_launch<true, int64_t>(b, st);
+        else small_launch<false, int64_t>(b, st);
     } else {
-        if (!count) small_launch<false, false, int32_t>(b, st);
-        else if (want_R) small_launch<true, true, int32_t>(b, st);
-        else small_launch<true, false, int32_t>(b, st);
+        if (want_R) small_launch<true, int32_t>(b, st);
+        else small_launch<false, int32_t>(b, st);
     }
 }
 
@@ -1579,12 +1576,10 @@ static void count_launch(const PipelineBufs &b, hipStream_t st)
                            b.pair_count, b.slots, b.slotsR, b.slot_cap, b.status);
 }
 
-void launch_small(const PipelineBufs &b, bool count, bool want_R, hipStream_t st);
-
 void launch_count(const PipelineBufs &b, bool want_R, bool wide, hipStream_t st)
 {
     if (b.nunits > 0) {   // cell list, counts and slot rows of every unit of small systems
-        launch_small(b, true, want_R, st);
+        launch_small(b, want_R, st);
         return;
     }
     if (b.nat == 0 || b.nbricks == 0) return;

@@ -467,18 +467,20 @@ def test_lazy_sweep_over_every_atom():
         nlhip.neighbours(cl, 100_001)
 
 
-def test_lazy_range_in_a_batch():
-    """The range query on one system of a batch (system-local indices)."""
+@pytest.mark.parametrize("nsys", [24, 96])
+def test_lazy_range_in_a_batch(nsys):
+    """The range query on one system of a batch (system-local indices); 96
+    systems take the fused small-system build, whose cell list it reads."""
     import ctypes as C
 
     from nlhip import _lib
     from nlhip.batch import BatchBuilder
 
-    systems = workloads.batch_systems(24, seed=151, nmax=700)
+    systems = workloads.batch_systems(nsys, seed=151, nmax=700)
     bb = BatchBuilder(np.int64)
     bb.run(systems, copy=False)
     L = _lib.lib()
-    for sidx in (0, 5, 23):
+    for sidx in (0, 5, nsys - 1):
         sysm = systems[sidx]
         ref = oracle.neighbour_list(sysm.X, sysm.cutoff, sysm.cell, sysm.pbc, ti_bits=64, want_R=True)
         n = sysm.nat
