@@ -637,7 +637,7 @@ __device__ __forceinline__ int64_t slot_row(int64_t e, int32_t cap)
 constexpr uint32_t kDropOffset = 0x40800000u;
 constexpr int32_t kRawBufferWord3 = 0x00020000; // raw 32-bit buffer access (GFX9 family)
 
-template <bool NARROW, bool WZERO, bool WANT_R, bool WIDE>
+template <bool NARROW, bool WZERO, bool WANT_R, bool WIDE, bool DENSE = false>
 __device__ __forceinline__ void emit_row(void *__restrict__ row, double *__restrict__ rrow, uint32_t cap,
                                          Status *__restrict__ status, uint64_t m, int32_t &ca, uint64_t ent,
                                          int32_t wix, int32_t wiy, int32_t wiz, double Rx, double Ry, double Rz)
@@ -647,7 +647,7 @@ __device__ __forceinline__ void emit_row(void *__restrict__ row, double *__restr
         // every lane stores, each lane without a hit at an offset past the
         // resource's end (a select on the ballot, not an exec-mask region);
         // an empty ballot skips it
-        if (n == 0) return;
+        if (!DENSE && n == 0) return;
         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(row, 0, (int)(cap * 4u), kRawBufferWord3);
         const uint32_t rk = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
         uint32_t off;
@@ -749,7 +749,7 @@ __device__ __forceinline__ void test_home_atom(int nch, const double *cx, const 
 // same shifts) against the same chunks: both distance tests of a chunk come
 // before both emissions, two independent chains the scheduler interleaves.
 // Atom 1's self candidate follows atom 0's.
-template <bool WANT_R, bool WIDE>
+template <bool WANT_R, bool WIDE, bool DENSE>
 __device__ __forceinline__ void test_home_pair(int nch, const double *cx, const double *cy, const double *cz,
                                                const double *sx, const double *sy, const double *sz, const uint64_t *csh,
                                                const uint64_t *ent, double h0x, double h0y, double h0z, double h1x,
@@ -779,8 +779,8 @@ __device__ __forceinline__ void test_home_pair(int nch, const double *cx, const 
             const bool in1 = (R1x * R1x + R1y * R1y) + R1z * R1z < rc2;
             const uint64_t m0 = __builtin_amdgcn_ballot_w64(in0) & (g == gs0 ? nself0 : ~0ull);
             const uint64_t m1 = __builtin_amdgcn_ballot_w64(in1) & (g == gs1 ? nself1 : ~0ull);
-            emit_row<true, true, WANT_R, WIDE>(row0, rrow0, cap, status, m0, ca0, ent[g], 0, 0, 0, R0x, R0y, R0z);
-            emit_row<true, true, WANT_R, WIDE>(row1, rrow1, cap, status, m1, ca1, ent[g], 0, 0, 0, R1x, R1y, R1z);
+            emit_row<true, true, WANT_R, WIDE, DENSE>(row0, rrow0, cap, status, m0, ca0, ent[g], 0, 0, 0, R0x, R0y, R0z);
+            emit_row<true, true, WANT_R, WIDE, DENSE>(row1, rrow1, cap, status, m1, ca1, ent[g], 0, 0, 0, R1x, R1y, R1z);
         }
     }
 }
@@ -792,7 +792,7 @@ __device__ __forceinline__ void test_home_pair(int nch, const double *cx, const 
 // atom a).  home: the nh home atoms in LDS; ht_l, hj_l: lane a holds
 // pack(-w_i) and j of home atom a; kpass: the self candidate of home atom 0
 // in this pass (home atom a's is kpass + a); Cm: the cell matrix.
-template <bool WANT_R, bool WIDE>
+template <bool WANT_R, bool WIDE, bool DENSE>
 __device__ __forceinline__ void pass_home_atoms(bool narrow, int nch, const double *cx, const double *cy,
                                                 const double *cz, const uint64_t *ent, const LAtom *home, int nh,
                                                 uint32_t ht_l, int32_t hj_l, int kpass, const double *Cm, double rc2,
@@ -838,7 +838,7 @@ __device__ __forceinline__ void pass_home_atoms(bool narrow, int nch, const doub
                               : (void *)(reinterpret_cast<uint64_t *>(slots) + srow1);
             double *rrow0 = WANT_R ? slotsR + 3 * srow0 : nullptr;
             double *rrow1 = WANT_R ? slotsR + 3 * srow1 : nullptr;
-            test_home_pair<WANT_R, WIDE>(nch, cx, cy, cz, sx, sy, sz, csh, ent, h0x, h0y, h0z, h1x, h1y,
+            test_home_pair<WANT_R, WIDE, DENSE>(nch, cx, cy, cz, sx, sy, sz, csh, ent, h0x, h0y, h0z, h1x, h1y,
                                          h1z, rc2, kpass + a, row0, row1, rrow0, rrow1,
                                          (uint32_t)cap, status, ca0, ca1);
             cnt_l = __builtin_amdgcn_inverse_ballot_w64(1ull << a) ? ca0 : cnt_l;
@@ -958,7 +958,7 @@ __global__ __launch_bounds__(BT) void count_kernel(
                     cz[g] = c.z;
                     ent[g] = (uint64_t)((uint32_t)c.j & jmask) | ((uint64_t)c.t << 32);
                 }
-                pass_home_atoms<WANT_R, WIDE>(narrow, nch, cx, cy, cz, ent, L.a + hb, nh, ht_l, hj_l, kself - k0, Csh,
+                pass_home_atoms<WANT_R, WIDE, true>(narrow, nch, cx, cy, cz, ent, L.a + hb, nh, ht_l, hj_l, kself - k0, Csh,
                                               rc2, sd.atom_base, slots, slotsR, cap, status, cnt_l);
             }
             if (lane < nh) pair_count[sd.atom_base + hj_l] = cnt_l | (narrow ? 0 : kRow8Bit);
@@ -1300,7 +1300,7 @@ __global__ __launch_bounds__(kSmallThreads) void small_kernel(SmallArgs A)
                         const uint32_t lo = narrow ? ((uint32_t)c.j | code << 26) : (uint32_t)c.j;
                         ent[g] = (uint64_t)lo | ((uint64_t)t << 32);
                     }
-                    pass_home_atoms<WANT_R, false>(narrow, nch, cx, cy, cz, ent, L.a + hb, nh, ht_l, hj_l,
+                    pass_home_atoms<WANT_R, false, false>(narrow, nch, cx, cy, cz, ent, L.a + hb, nh, ht_l, hj_l,
                                                    self0 + hg - k0, sd.C, sd.rc2, sd.atom_base, A.slots, A.slotsR,
                                                    A.cap, A.status, cnt_l);
                 }

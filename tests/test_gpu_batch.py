@@ -289,3 +289,32 @@ def test_fused_small_tiny_cells_and_empty_systems():
     plain = BatchBuilder()
     _fused_off(plain)
     _same(plain.run(systems, want_R=True), res)
+
+
+def test_fused_small_rows_and_outputs_regrow():
+    """Clumped systems: every atom sees all others, far above the density
+    estimate the slot rows and pair outputs are first sized for, so the fused
+    build regrows both and counts again; equal to the general pipeline and
+    the oracle.  A rebuild with spread-out positions then fits the grown
+    buffers."""
+    rng = np.random.default_rng(184)
+    systems = []
+    for k in range(80):
+        n = 40 + (k * 37) % 260
+        side = 18.0 + 4.0 * rng.random()
+        C = np.diag([side, side, side])
+        X = np.array([side / 2] * 3) + 2.5 * (rng.random((n, 3)) - 0.5)
+        systems.append(workloads.System(X, C, (bool(k % 2), True, bool(k % 3)), 4.5 + rng.random()))
+    X, offs = pack_systems(systems)
+    bb = BatchBuilder()
+    print("fused clumped build", flush=True)
+    res = bb.run(systems, want_R=True)
+    _check_oracle(res, systems, X, offs, [0, 1, 17, 40, 79])
+    plain = BatchBuilder()
+    _fused_off(plain)
+    _same(plain.run(systems, want_R=True), res)
+    spread = [workloads.System(rng.random((s.nat, 3)) @ s.cell, s.cell, s.pbc, s.cutoff) for s in systems]
+    X2, _ = pack_systems(spread)
+    print("fused rebuild spread", flush=True)
+    g = bb.rebuild(X2, want_R=True)
+    _check_oracle(g, spread, X2, offs, [0, 3, 79])
