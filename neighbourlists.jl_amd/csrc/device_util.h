@@ -10,25 +10,55 @@ namespace nlh {
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 
+// Inclusive scans across lanes by DPP (data-parallel primitives: the VALU
+// reads a neighbouring lane's register in the same instruction) instead of
+// ds_bpermute round trips through the LDS crossbar.  GFX9 DPP controls:
+// row_shr:n = 0x110 + n shifts within each 16-lane row, lanes shifted in
+// from outside the row read 0; row_bcast:15 (0x142) hands lane 15 of a row
+// to the next row (row mask 0xa: rows 1 and 3), row_bcast:31 (0x143) hands
+// lane 31 to rows 2 and 3 (row mask 0xc).  Rows left out by the row mask
+// read 0.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ int dpp_or_zero(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, 0xf, false);
+}
+
+// Inclusive scan within each 16-lane row (lanes 0..15 of the wave: the
+// first row).
+__device__ __forceinline__ int row16_incl_scan(int v)
+{
+    v += dpp_or_zero<0x111>(v);
+    v += dpp_or_zero<0x112>(v);
+    v += dpp_or_zero<0x114>(v);
+    v += dpp_or_zero<0x118>(v);
+    return v;
+}
+
 __device__ __forceinline__ int wave_incl_scan(int v)
 {
-    const int lane = lane_id();
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        const int t = __shfl_up(v, off, kWave);
-        if (lane >= off) v += t;
-    }
+    v = row16_incl_scan(v);
+    v += dpp_or_zero<0x142, 0xa>(v);
+    v += dpp_or_zero<0x143, 0xc>(v);
     return v;
+}
+
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ int64_t dpp64_or_zero(int64_t v)
+{
+    const uint32_t lo = (uint32_t)dpp_or_zero<CTRL, ROW_MASK>((int)(uint32_t)(uint64_t)v);
+    const uint32_t hi = (uint32_t)dpp_or_zero<CTRL, ROW_MASK>((int)(uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
 __device__ __forceinline__ int64_t wave_incl_scan64(int64_t v)
 {
-    const int lane = lane_id();
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        const int64_t t = __shfl_up(v, off, kWave);
-        if (lane >= off) v += t;
-    }
+    v += dpp64_or_zero<0x111>(v);
+    v += dpp64_or_zero<0x112>(v);
+    v += dpp64_or_zero<0x114>(v);
+    v += dpp64_or_zero<0x118>(v);
+    v += dpp64_or_zero<0x142, 0xa>(v);
+    v += dpp64_or_zero<0x143, 0xc>(v);
     return v;
 }
 

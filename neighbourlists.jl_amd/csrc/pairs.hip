@@ -543,13 +543,8 @@ __device__ __forceinline__ HomeRows brick_home(const Brick &B, const BrickLDS<BT
         st = L.hoff[hr];
         len = L.hoff[hr + 3] - st;
     }
-    // inclusive scan over lanes 0..8 (four steps reach back 15 lanes)
-    int incl = len;
-#pragma unroll
-    for (int off = 1; off < 16; off <<= 1) {
-        const int t = __shfl_up(incl, off, kWave);
-        if (lane >= off) incl += t;
-    }
+    // inclusive scan over lanes 0..8 (within the first 16-lane row)
+    const int incl = row16_incl_scan(len);
     R.rex = incl - len;
     R.rstart = st;
     R.len = len;
@@ -1476,11 +1471,7 @@ __global__ __launch_bounds__(1024) void fill_kernel(
         const bool vk = lane < span;
         const int32_t pck = vk ? pair_count[ak] : 0;
         const int32_t cntk = pck & kCountMask;
-        int32_t incl = cntk;
-        for (int o = 1; o < span; o <<= 1) {
-            const int32_t t = __shfl_up(incl, o, kWave);
-            if (lane >= o) incl += t;
-        }
+        const int32_t incl = wave_incl_scan(cntk);   // lanes past the span hold 0
         // runs past the tile end keep their exact start (<= TP + 64 cap)
         const int32_t fo = vk ? incl - cntk - skip0 : kBeyond;
         const uint64_t row8 = __ballot((pck & kRow8Bit) != 0);
