@@ -337,6 +337,25 @@ struct BrickShape {
 constexpr uint32_t kPackZero = (512u) | (512u << 10) | (512u << 20);   // pack_wind(0, 0, 0)
 constexpr int kBrickChunks = 3;   // candidate chunks a lane holds in the brick pass (192 candidates)
 
+// Quotient of 0 <= x < 2^16 by 1 <= d < 2^16 through a reciprocal: m =
+// floor((2^32 - 1) / d) + 1 (0 for d = 1), q = umulhi(x, m).  Exact: m
+// exceeds 2^32 / d by less than one, so the product overshoots x / d by less
+// than x / 2^32 < 1 / d, which never carries past the next integer.  A few
+// instructions instead of the integer-division sequence.
+struct SmallDiv {
+    uint32_t d, m;
+};
+
+__device__ __forceinline__ SmallDiv small_div(int d)
+{
+    return {(uint32_t)d, d == 1 ? 0u : 0xffffffffu / (uint32_t)d + 1u};
+}
+
+__device__ __forceinline__ int small_quot(int x, SmallDiv D)
+{
+    return D.m ? (int)__umulhi((uint32_t)x, D.m) : x;
+}
+
 struct Brick {
     const SysDev *sd;
     int b0[3];     // 1-based coordinates of the first home cell
@@ -344,6 +363,7 @@ struct Brick {
     int hd[3];     // halo extent per axis (nh + 2)
     int nhalo, nhome;
     bool fast;     // one-cell reach and a small enough halo: the LDS path applies
+    SmallDiv dnh0, dnh1, dhd0, dhd1;   // divisors nh[0], nh[1], hd[0], hd[1]
 };
 
 __device__ __forceinline__ void brick_locate(const PairsArgs &A, const int32_t *__restrict__ sys_brick_base, int b,
@@ -364,6 +384,19 @@ __device__ __forceinline__ void brick_locate(const PairsArgs &A, const int32_t *
     B.nhalo = B.hd[0] * B.hd[1] * B.hd[2];
     B.nhome = B.nh[0] * B.nh[1] * B.nh[2];
     B.fast = sd.nxyz[0] == 1 && sd.nxyz[1] == 1 && sd.nxyz[2] == 1 && B.nhalo <= halo_max;
+    B.dnh0 = small_div(B.nh[0]);
+    B.dnh1 = small_div(B.nh[1]);
+    B.dhd0 = small_div(B.hd[0]);
+    B.dhd1 = small_div(B.hd[1]);
+}
+
+// wrap_and_shift (cell_list.jl:105-120) of an index within one cell of
+// [1, n] on a periodic axis (a halo cell of a one-cell-reach brick): at most
+// one wrap, no division.
+__device__ __forceinline__ void wrap_near(int32_t a, int32_t n, int32_t &w, int32_t &s)
+{
+    s = a < 1 ? -1 : (a > n ? 1 : 0);
+    w = a - s * n;
 }
 
 __device__ __forceinline__ int brick_cell_id(const Brick &B, int hc)
@@ -419,7 +452,8 @@ __device__ bool brick_load(const PairsArgs &A, const Brick &B, BrickLDS<BT> &L)
         cnt[u] = 0;
         const int h = tid + u;
         if (h < B.nhalo) {
-            const int ux = h % B.hd[0], uy = (h / B.hd[0]) % B.hd[1], uz = h / (B.hd[0] * B.hd[1]);
+            const int q0 = small_quot(h, B.dhd0), uz = small_quot(q0, B.dhd1);
+            const int ux = h - q0 * B.hd[0], uy = q0 - uz * B.hd[1];
             const int a[3] = {B.b0[0] - 1 + ux, B.b0[1] - 1 + uy, B.b0[2] - 1 + uz};
             bool inb = true;
             for (int k = 0; k < 3; ++k) inb = inb && (p[k] || (a[k] >= 1 && a[k] <= sd.n[k]));
@@ -427,7 +461,9 @@ __device__ bool brick_load(const PairsArgs &A, const Brick &B, BrickLDS<BT> &L)
             uint32_t sh = kPackZero;
             if (inb) {   // _is_cell_in_bounds / _get_neighbor_cell (gpu_kernels.jl:20-47)
                 int32_t c[3], sft[3];
-                for (int k = 0; k < 3; ++k) wrap_and_shift(a[k], sd.n[k], p[k], c[k], sft[k]);
+                // a halo cell lies within one cell of the grid: an open axis
+                // holds it in bounds here, a periodic one wraps it at most once
+                for (int k = 0; k < 3; ++k) wrap_near(a[k], sd.n[k], c[k], sft[k]);
                 const int gc = sd.cell_base + (c[0] - 1) + (c[1] - 1) * sd.n[0] + (c[2] - 1) * sd.n[0] * sd.n[1];
                 st = A.cell_start[gc];
                 cnt[u] = A.cell_start[gc + 1] - st;
@@ -530,7 +566,8 @@ template <int BT>
 __device__ __forceinline__ HomeRows brick_home(const Brick &B, const BrickLDS<BT> &L, int hc)
 {
     HomeRows R;
-    const int ix = hc % B.nh[0], iy = (hc / B.nh[0]) % B.nh[1], iz = hc / (B.nh[0] * B.nh[1]);
+    const int q0 = small_quot(hc, B.dnh0), iz = small_quot(q0, B.dnh1);
+    const int ix = hc - q0 * B.nh[0], iy = q0 - iz * B.nh[1];
     const int hh = (ix + 1) + B.hd[0] * ((iy + 1) + B.hd[1] * (iz + 1));
     R.h0 = sgpr(L.hoff[hh]);
     R.h1 = sgpr(L.hoff[hh + 1]);
