@@ -817,6 +817,34 @@ __device__ __forceinline__ void test_home_pair(int nch, const double *cx, const 
     }
 }
 
+// test_home_pair for three full chunks with no shift and 4-byte rows without
+// R: the six distance tests of the pair, then its six emissions (every lane
+// stores, lanes without a hit at the dropped offset), one branch-free block.
+__device__ __forceinline__ void test_pair_plain(const double *cx, const double *cy, const double *cz,
+                                                const uint64_t *ent, double h0x, double h0y, double h0z, double h1x,
+                                                double h1y, double h1z, double rc2, int ks0, void *__restrict__ row0,
+                                                void *__restrict__ row1, uint32_t cap, int32_t &ca0, int32_t &ca1)
+{
+    const int gs0 = ks0 >> 6, gs1 = (ks0 + 1) >> 6;
+    const uint64_t nself0 = ~(1ull << (ks0 & (kWave - 1))), nself1 = ~(1ull << ((ks0 + 1) & (kWave - 1)));
+    uint64_t m0[kBrickChunks], m1[kBrickChunks];
+#pragma unroll
+    for (int g = 0; g < kBrickChunks; ++g) {
+        // R = xj - xi (shift zero) ; r^2 = dot(R, R)  (gpu_kernels.jl:89-92)
+        const double R0x = cx[g] - h0x, R0y = cy[g] - h0y, R0z = cz[g] - h0z;
+        const double R1x = cx[g] - h1x, R1y = cy[g] - h1y, R1z = cz[g] - h1z;
+        const bool in0 = (R0x * R0x + R0y * R0y) + R0z * R0z < rc2;
+        const bool in1 = (R1x * R1x + R1y * R1y) + R1z * R1z < rc2;
+        m0[g] = __builtin_amdgcn_ballot_w64(in0) & (g == gs0 ? nself0 : ~0ull);
+        m1[g] = __builtin_amdgcn_ballot_w64(in1) & (g == gs1 ? nself1 : ~0ull);
+    }
+#pragma unroll
+    for (int g = 0; g < kBrickChunks; ++g) {
+        emit_row<true, true, false, false, true>(row0, nullptr, cap, nullptr, m0[g], ca0, ent[g], 0, 0, 0, 0.0, 0.0, 0.0);
+        emit_row<true, true, false, false, true>(row1, nullptr, cap, nullptr, m1[g], ca1, ent[g], 0, 0, 0, 0.0, 0.0, 0.0);
+    }
+}
+
 // One pass of a home cell's atoms over loaded candidate chunks: every home
 // atom's distance tests against the nch chunks (cx, cy, cz, ent: j |
 // pack(s - w_j) << 32, or j | S code << 26 in a 4-byte brick), its hits
@@ -856,6 +884,19 @@ __device__ __forceinline__ void pass_home_atoms(bool narrow, int nch, const doub
             }
         }
         wcur = kPackZero;
+        if (DENSE && !WANT_R && !WIDE && nch == kBrickChunks && (csh[0] | csh[1] | csh[2]) == 0) {
+            for (; a + 1 < nh; a += 2) {
+                const LAtom h0 = home[a], h1 = home[a + 1];
+                int32_t ca0 = __builtin_amdgcn_readlane(cnt_l, a), ca1 = __builtin_amdgcn_readlane(cnt_l, a + 1);
+                const int64_t srow0 = slot_row(atom_base + __builtin_amdgcn_readlane(hj_l, a), cap);
+                const int64_t srow1 = slot_row(atom_base + __builtin_amdgcn_readlane(hj_l, a + 1), cap);
+                test_pair_plain(cx, cy, cz, ent, h0.x, h0.y, h0.z, h1.x, h1.y, h1.z, rc2, kpass + a,
+                                reinterpret_cast<uint64_t *>(slots) + srow0, reinterpret_cast<uint64_t *>(slots) + srow1,
+                                (uint32_t)cap, ca0, ca1);
+                cnt_l = __builtin_amdgcn_inverse_ballot_w64(1ull << a) ? ca0 : cnt_l;
+                cnt_l = __builtin_amdgcn_inverse_ballot_w64(2ull << a) ? ca1 : cnt_l;
+            }
+        }
         for (; a + 1 < nh; a += 2) {
             // uniform LDS reads: every lane receives the home atom's coordinates
             const LAtom h0 = home[a], h1 = home[a + 1];
