@@ -581,6 +581,36 @@ __device__ __forceinline__ int2 load_ke(const int2 *p)
 // index, the order of Julia's stable sortperm.  A bucket of at most
 // kBucketLdsAtoms atoms keeps its cells, indices and groups in LDS; a larger
 // one (clustered atoms) works from global memory (bke and tmp, its own range).
+// The sorted record, perm and cell_id of atom e (global index) of cell k at
+// sorted place p.
+template <typename TI>
+__device__ __forceinline__ void place_one(int32_t k, int32_t e, int32_t p, const double *__restrict__ X,
+                                          const SysDev *__restrict__ sys, const int32_t *__restrict__ cell_sys, int nsys,
+                                          Rec *__restrict__ rec, int32_t *__restrict__ wbig, void *__restrict__ perm_out,
+                                          void *__restrict__ cell_id_out)
+{
+    const int si = nsys == 1 ? 0 : cell_sys[k];
+    const SysDev &sd = sys[si];
+    const double x = X[3 * (int64_t)e], y = X[3 * (int64_t)e + 1], z = X[3 * (int64_t)e + 2];
+    int32_t cc[3], w[3];
+    cell_and_winding(sd, x, y, z, cc, w);
+    const int32_t j = (int32_t)(e - sd.atom_base);
+    Rec o;
+    o.x = x;
+    o.y = y;
+    o.z = z;
+    o.j = j;
+    o.w = pack_wind(w[0], w[1], w[2]);
+    rec[p] = o;
+    if (o.w & kWindBig) {
+        wbig[3 * (int64_t)p] = w[0];
+        wbig[3 * (int64_t)p + 1] = w[1];
+        wbig[3 * (int64_t)p + 2] = w[2];
+    }
+    store_ti<TI>(perm_out, p, (int64_t)j + 1);
+    store_ti<TI>(cell_id_out, p, (int64_t)(k - sd.cell_base) + 1);
+}
+
 template <typename TI>
 __global__ __launch_bounds__(kBucketThreads) void bucket_place_kernel(
     const int2 *__restrict__ bke, const double *__restrict__ X, const int32_t *__restrict__ boff, SortPlan P,
@@ -656,12 +686,43 @@ __global__ __launch_bounds__(kBucketThreads) void bucket_place_kernel(
         else grpg[q] = e;
     }
     __syncthreads();   // the groups (LDS, or the block's own tmp stores) are visible to the block
+    if (small) {
+        // each atom's place in the bucket (its cell's start plus the atoms
+        // of its cell with a smaller index), then (cell, index) moved to that
+        // place in LDS, so that the records go out in sorted order: whole
+        // cache lines instead of scattered 32-byte stores
+        constexpr int kPer = kBucketLdsAtoms / kBucketThreads;
+        int32_t kq[kPer], eq[kPer], pq[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int i = u * kBucketThreads + (int)threadIdx.x;
+            pq[u] = -1;
+            if (i < n) {
+                const int32_t k = kl[i], e = el[i];
+                const int c = k - c0;
+                const int s0 = co[c], s1 = co[c + 1];
+                int32_t rank = 0;
+                for (int t = s0; t < s1; ++t) rank += grp[t] < e;
+                kq[u] = k;
+                eq[u] = e;
+                pq[u] = s0 + rank;
+            }
+        }
+        __syncthreads();   // every read of kl and el is done
+#pragma unroll
+        for (int u = 0; u < kPer; ++u)
+            if (pq[u] >= 0) {
+                kl[pq[u]] = kq[u];
+                el[pq[u]] = eq[u];
+            }
+        __syncthreads();
+        for (int q = threadIdx.x; q < n; q += kBucketThreads)
+            place_one<TI>(kl[q], el[q], bs + q, X, sys, cell_sys, nsys, rec, wbig, perm_out, cell_id_out);
+        return;
+    }
     for (int i = threadIdx.x; i < n; i += kBucketThreads) {
         int32_t k, e;
-        if (small) {
-            k = kl[i];
-            e = el[i];
-        } else {
+        {
             const int2 ke = load_ke(bke + bs + i);
             k = ke.x;
             e = ke.y;
@@ -669,31 +730,9 @@ __global__ __launch_bounds__(kBucketThreads) void bucket_place_kernel(
         const int c = k - c0;
         const int s0 = co[c], s1 = co[c + 1];
         int32_t rank = 0;
-        if (small)
-            for (int t = s0; t < s1; ++t) rank += grp[t] < e;
-        else
-            for (int t = s0; t < s1; ++t) rank += grpg[t] < e;
+        for (int t = s0; t < s1; ++t) rank += grpg[t] < e;
         const int32_t p = bs + s0 + rank;
-        const int si = nsys == 1 ? 0 : cell_sys[k];
-        const SysDev &sd = sys[si];
-        const double x = X[3 * (int64_t)e], y = X[3 * (int64_t)e + 1], z = X[3 * (int64_t)e + 2];
-        int32_t cc[3], w[3];
-        cell_and_winding(sd, x, y, z, cc, w);
-        const int32_t j = (int32_t)(e - sd.atom_base);
-        Rec o;
-        o.x = x;
-        o.y = y;
-        o.z = z;
-        o.j = j;
-        o.w = pack_wind(w[0], w[1], w[2]);
-        rec[p] = o;
-        if (o.w & kWindBig) {
-            wbig[3 * (int64_t)p] = w[0];
-            wbig[3 * (int64_t)p + 1] = w[1];
-            wbig[3 * (int64_t)p + 2] = w[2];
-        }
-        store_ti<TI>(perm_out, p, (int64_t)j + 1);
-        store_ti<TI>(cell_id_out, p, (int64_t)(k - sd.cell_base) + 1);
+        place_one<TI>(k, e, p, X, sys, cell_sys, nsys, rec, wbig, perm_out, cell_id_out);
     }
 }
 
