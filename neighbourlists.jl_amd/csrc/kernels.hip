@@ -465,7 +465,8 @@ __global__ __launch_bounds__(kBinThreads) void bucket_count_kernel(
     extern __shared__ int32_t hist[];   // P.nb counters
     for (int b = threadIdx.x; b < P.nb; b += kBinThreads) hist[b] = 0;
     __syncthreads();
-    const int64_t a0 = (int64_t)blockIdx.x * P.apb;
+    const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);   // consecutive blocks on one XCD
+    const int64_t a0 = (int64_t)lb * P.apb;
     const int64_t a1 = a0 + P.apb < nat ? a0 + P.apb : nat;
     bool bad = false;
     for (int64_t g0 = a0 + threadIdx.x; g0 < a1; g0 += kBinThreads * kBinUnroll) {
@@ -501,7 +502,7 @@ __global__ __launch_bounds__(kBinThreads) void bucket_count_kernel(
     }
     if (bad) atomicOr(&status->bad_position, 1);
     __syncthreads();
-    for (int b = threadIdx.x; b < P.nb; b += kBinThreads) bmat[(int64_t)b * P.na + blockIdx.x] = hist[b];
+    for (int b = threadIdx.x; b < P.nb; b += kBinThreads) bmat[(int64_t)b * P.na + lb] = hist[b];
 }
 
 // Exclusive scan of n counts in place (the bucket counts); tile prefixes
@@ -546,7 +547,8 @@ __global__ __launch_bounds__(kBinThreads) void bucket_scatter_kernel(int64_t nat
                                                                      const int32_t *__restrict__ boff,
                                                                      int2 *__restrict__ bke)
 {
-    const int64_t a0 = (int64_t)blockIdx.x * P.apb;
+    const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);   // the count kernel's block order
+    const int64_t a0 = (int64_t)lb * P.apb;
     const int64_t a1 = a0 + P.apb < nat ? a0 + P.apb : nat;
     for (int64_t g0 = a0 + threadIdx.x; g0 < a1; g0 += kBinThreads * kBinUnroll) {
         int32_t k[kBinUnroll], pos[kBinUnroll];
@@ -564,7 +566,7 @@ __global__ __launch_bounds__(kBinThreads) void bucket_scatter_kernel(int64_t nat
         for (int u = 0; u < kBinUnroll; ++u) {
             const int64_t g = g0 + (int64_t)u * kBinThreads;
             // scattered stores: cached, so that the L2 merges them into lines
-            if (g < a1) bke[boff[(int64_t)(k[u] >> P.bw) * P.na + blockIdx.x] + pos[u]] = make_int2(k[u], (int32_t)g);
+            if (g < a1) bke[boff[(int64_t)(k[u] >> P.bw) * P.na + lb] + pos[u]] = make_int2(k[u], (int32_t)g);
         }
     }
 }

@@ -555,7 +555,6 @@ __device__ bool brick_load(const PairsArgs &A, const Brick &B, BrickLDS<BT> &L)
 // the home cell's first atom (its own self candidate).
 struct HomeRows {
     int h0, h1;      // LDS range of the home atoms (uniform)
-    int pbase;       // sorted index of the first home atom (uniform)
     int rstart, rex; // per lane: LDS start and exclusive candidate prefix of row `lane`
     int len;         // per lane: candidates in row `lane`
     int total;       // candidates (uniform)
@@ -571,7 +570,6 @@ __device__ __forceinline__ HomeRows brick_home(const Brick &B, const BrickLDS<BT
     const int hh = (ix + 1) + B.hd[0] * ((iy + 1) + B.hd[1] * (iz + 1));
     R.h0 = sgpr(L.hoff[hh]);
     R.h1 = sgpr(L.hoff[hh + 1]);
-    R.pbase = sgpr(L.hstart[hh]);
     const int lane = lane_id();
     int len = 0, st = 0;
     if (lane < 9) {   // row (dz, dy) = (lane / 3 - 1, lane % 3 - 1); dx = -1..1 contiguous
