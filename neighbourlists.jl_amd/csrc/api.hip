@@ -504,7 +504,7 @@ void assign_bricks(nlh_handle *h)
 // beyond +-511 made a fused build fall back) no units: the general pipeline.
 constexpr int kSmallMinSystems = 64;
 constexpr int kMaxUnitParts = 16;       // parts of one unit at most (each at least 4 home cells)
-constexpr double kItemRounds = 3.0;     // work items per CU the split aims at
+constexpr double kItemRounds = 2.0;     // work items per CU the split aims at
 
 // Estimated count work of a system in distance tests of 64 candidates: every
 // atom against the atoms of its (2 nxyz + 1)^3 neighbour cells, the cells
