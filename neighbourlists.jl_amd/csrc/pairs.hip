@@ -1046,10 +1046,10 @@ __global__ __launch_bounds__(BT) void count_kernel(
 // kUnitSystems systems) or one of its parts, all of it in LDS: the cell and
 // winding of every atom, a stable counting sort by cell (ties in ascending
 // original index, Julia's stable sortperm), the cell-list outputs (written
-// by part 0), and the neighbour count and slot row of every
-// atom in the part's home cells (unit cells part, part + parts, ...), through
-// the brick pass's tests and emission; the waves take those home cells from
-// a workgroup counter.  A home cell's
+// by part 0), and the neighbour count and slot row of every home atom of
+// the part's count work items (items part, part + parts, ..., each a group
+// of home atoms of one cell), through the brick pass's tests and emission;
+// the waves take those items from a workgroup counter.  A home cell's
 // candidates are its (2 nxyz + 1)^3 neighbour cells in the reference order
 // (dz, dy, dx outer to inner, gpu_kernels.jl:69-71), each a segment of the
 // sorted atoms with its cell shift, taken 64 segments at a time.
@@ -1143,7 +1143,7 @@ __global__ __launch_bounds__(kSmallThreads) void small_kernel(SmallArgs A)
     __shared__ SmallLDS L;
     const int tid = (int)threadIdx.x, lane = lane_id(), wv = sgpr(tid >> 6);
     const int u = A.order[2 * blockIdx.x], pinfo = A.order[2 * blockIdx.x + 1];
-    const int part = pinfo & 0xffff, parts = pinfo >> 16;   // this workgroup counts home cells part + parts k
+    const int part = pinfo & 0xffff, parts = pinfo >> 16;   // this workgroup counts items part + parts k
     const bool writer = part == 0;                           // the first part writes the cell list
     const int s0 = A.units[u], ns = A.units[u + 1] - s0;
     const int64_t A0 = A.sys_atom_base[s0];
