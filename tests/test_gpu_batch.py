@@ -318,3 +318,34 @@ def test_fused_small_rows_and_outputs_regrow():
     print("fused rebuild spread", flush=True)
     g = bb.rebuild(X2, want_R=True)
     _check_oracle(g, spread, X2, offs, [0, 3, 79])
+
+
+def test_fused_small_unit_limits():
+    """Unit edges of the fused build: 300 one-atom systems (a unit holds at
+    most 256 systems), a system of exactly 2048 atoms, and a rod of 256
+    cells along one axis (the most a system may have); then the same batch
+    with a 257-cell rod, which the general pipeline builds.  Every case
+    equals the oracle and the general pipeline."""
+    rng = np.random.default_rng(185)
+    systems = []
+    for k in range(300):
+        side = 6.0 + 2.0 * rng.random()
+        systems.append(workloads.System(rng.random((1, 3)) * side, np.eye(3) * side, (True, bool(k % 2), True), 5.0))
+    d = (2048 / 0.06) ** (1.0 / 3.0)
+    systems.append(workloads.System(rng.random((2048, 3)) * d, np.eye(3) * d, (True, True, False), 5.0))
+
+    def rod(ncell):
+        C = np.diag([5.0 * ncell + 1.0, 6.0, 6.0])
+        return workloads.System(rng.random((400, 3)) @ C, C, (True, False, True), 5.0)
+
+    for tail in (rod(256), rod(257)):
+        batch = systems + [tail]
+        g = _lib.geometry(tail.cell, tail.cutoff, tail.pbc, 4)
+        print(f"fused unit limits, rod of {g.ncells[0]} cells", flush=True)
+        X, offs = pack_systems(batch)
+        bb = BatchBuilder()
+        res = bb.run(batch, want_R=True)
+        _check_oracle(res, batch, X, offs, [0, 255, 256, 299, 300, 301])
+        plain = BatchBuilder()
+        _fused_off(plain)
+        _same(plain.run(batch, want_R=True), res)
