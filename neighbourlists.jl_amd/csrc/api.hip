@@ -458,6 +458,41 @@ void choose_brick(const nlh_geometry &g, int64_t nat, int32_t budget, int32_t ha
     if (!found) out[0] = out[1] = out[2] = 1;
 }
 
+// Brick classes of a system whose brick shape is set: per class its first
+// brick in launch order and its shape (see SysDev), largest bricks first, so
+// that the bricks that take longest start first and the short ones fill the
+// end of the launch.  Returns the system's bricks.
+int64_t brick_classes(nlh::SysDev &sd)
+{
+    struct Cls {
+        int64_t cells, bricks;
+        int32_t mask;
+    };
+    Cls cl[8];
+    int ncl = 0;
+    for (int k = 0; k < 3; ++k) sd.bfull[k] = sd.n[k] / sd.brick[k];
+    for (int32_t m = 0; m < 8; ++m) {
+        int64_t cells = 1, bricks = 1;
+        for (int k = 0; k < 3; ++k) {
+            const int32_t rem = sd.n[k] - sd.bfull[k] * sd.brick[k];
+            cells *= (m >> k & 1) ? rem : sd.brick[k];
+            bricks *= (m >> k & 1) ? (rem > 0 ? 1 : 0) : sd.bfull[k];
+        }
+        if (bricks > 0) cl[ncl++] = {cells, bricks, m};
+    }
+    std::stable_sort(cl, cl + ncl, [](const Cls &a, const Cls &b) { return a.cells > b.cells; });
+    int64_t first = 0;
+    for (int c = 0; c < 8; ++c) {
+        if (first > INT32_MAX - 1) throw Error(NLH_EOVERFLOW, "too many pair-search bricks");
+        sd.cls_first[c] = (int32_t)first;
+        sd.cls_mask[c] = c < ncl ? cl[c].mask : 0;
+        if (c < ncl) first += cl[c].bricks;
+    }
+    if (first > INT32_MAX - 1) throw Error(NLH_EOVERFLOW, "too many pair-search bricks");
+    sd.cls_first[8] = (int32_t)first;
+    return first;
+}
+
 // Repeated single builds replay the captured pipeline below this size.
 constexpr int64_t kReplayMaxAtoms = 12288;
 
@@ -486,8 +521,7 @@ void assign_bricks(nlh_handle *h)
                      sd.brick, min_bricks);
         sd.brick_base = (int32_t)bricks;
         h->brick_base_host[s] = (int32_t)bricks;
-        bricks += (int64_t)((sd.n[0] + sd.brick[0] - 1) / sd.brick[0]) * ((sd.n[1] + sd.brick[1] - 1) / sd.brick[1]) *
-                  ((sd.n[2] + sd.brick[2] - 1) / sd.brick[2]);
+        bricks += brick_classes(sd);
         if (bricks > INT32_MAX - 1) throw Error(NLH_EOVERFLOW, "too many pair-search bricks in one batch");
     }
     h->brick_base_host[nsys] = (int32_t)bricks;
@@ -1426,9 +1460,7 @@ void neighbour_list_noalloc(hipStream_t st, const double *X, int64_t nat, const 
     choose_brick(g, nat, std::min<int32_t>(920, (int32_t)(kHaloFill * 4 * 256)), 256, P.sd.brick,
                  2 * (int64_t)cus * 4);
     P.sd.brick_base = 0;
-    P.nbricks = (int64_t)((P.sd.n[0] + P.sd.brick[0] - 1) / P.sd.brick[0]) *
-                ((P.sd.n[1] + P.sd.brick[1] - 1) / P.sd.brick[1]) * ((P.sd.n[2] + P.sd.brick[2] - 1) / P.sd.brick[2]);
-    if (P.nbricks > INT32_MAX - 1) throw Error(NLH_EOVERFLOW, "too many pair-search bricks");
+    P.nbricks = brick_classes(P.sd);
     P.atom_base[0] = 0;
     P.atom_base[1] = nat;
     P.cell_base[0] = 0;

@@ -93,6 +93,15 @@ struct SysDev {
     int64_t first_base; // offset of this system's `first` array (atom_base + sys index)
     int32_t brick[3];   // home cells per brick along each axis (pair search work unit)
     int32_t brick_base; // first global brick index
+    // Launch order of the bricks: classes of one brick shape (along each
+    // axis a full brick, or the partial last one of an axis that brick[k]
+    // does not divide), in decreasing home cells, each class in row-major
+    // order.  Class c holds the bricks [cls_first[c], cls_first[c + 1]) of
+    // the system's launch order; cls_first[c] is the system's brick count
+    // for every c past the last class.
+    int32_t bfull[3];      // full bricks along each axis (n / brick)
+    int32_t cls_first[9];
+    int32_t cls_mask[8];   // bit k set: the class's bricks are the partial last brick along axis k
 };
 
 // Sorted atom record: position, original (system-local, 0-based) index and

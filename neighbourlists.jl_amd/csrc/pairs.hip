@@ -366,16 +366,29 @@ struct Brick {
     SmallDiv dnh0, dnh1, dhd0, dhd1;   // divisors nh[0], nh[1], hd[0], hd[1]
 };
 
-__device__ __forceinline__ void brick_locate(const PairsArgs &A, const int32_t *__restrict__ sys_brick_base, int b,
+// Brick of workgroup w of a count launch: the bricks of the launch in
+// system order, each system's in its class order, largest first
+// (SysDev::cls_first).  Workgroups are dealt to the eight XCDs in turn, so
+// every XCD takes its share of each class and of every region of a system
+// (a slab with vacuum above it puts its empty bricks on every XCD, not on
+// some of them).
+__device__ __forceinline__ void brick_locate(const PairsArgs &A, const int32_t *__restrict__ sys_brick_base, int w,
                                              int halo_max, Brick &B)
 {
-    const int s = A.nsys == 1 ? 0 : find_sys(sys_brick_base, A.nsys, (int32_t)b);
+    const int s = A.nsys == 1 ? 0 : find_sys(sys_brick_base, A.nsys, (int32_t)w);
     const SysDev &sd = A.sys[s];
     B.sd = &sd;
-    const int lb = b - sys_brick_base[s];
-    const int nb0 = (sd.n[0] + sd.brick[0] - 1) / sd.brick[0];
-    const int nb1 = (sd.n[1] + sd.brick[1] - 1) / sd.brick[1];
-    const int ib[3] = {lb % nb0, (lb / nb0) % nb1, lb / (nb0 * nb1)};
+    int li = w - sys_brick_base[s], c = 0;
+#pragma unroll
+    for (int cc = 1; cc < 8; ++cc) c = li >= sd.cls_first[cc] ? cc : c;
+    li -= sd.cls_first[c];
+    // brick (ib0, ib1, ib2) of the class: the partial last brick along the
+    // axes of its mask, row-major over the full bricks along the others
+    const int m = sd.cls_mask[c];
+    const int d0 = (m & 1) ? 1 : sd.bfull[0], d1 = (m & 2) ? 1 : sd.bfull[1];
+    const int q0 = li / d0;
+    const int ib[3] = {(m & 1) ? sd.bfull[0] : li - q0 * d0, (m & 2) ? sd.bfull[1] : q0 % d1,
+                       (m & 4) ? sd.bfull[2] : q0 / d1};
     for (int k = 0; k < 3; ++k) {
         B.b0[k] = ib[k] * sd.brick[k] + 1;
         B.nh[k] = min(sd.brick[k], sd.n[k] - B.b0[k] + 1);
@@ -662,9 +675,12 @@ __device__ __forceinline__ int64_t slot_row(int64_t e, int32_t cap)
 // entries: every lane stores, lanes without a hit at an offset past the
 // resource's end, and the hardware's range check drops those stores and any
 // entry at r >= cap.  No exec-mask or bound branches.
-// beyond any row resource (cap * 4 < 2^30); the bit pattern of 4.0f, an
-// inline constant, so the select that applies it needs no literal
+//
+// The offset of a lane without a hit: beyond any row resource (cap * 4 <
+// 2^30); the bit pattern of 4.0f, an inline constant, so the select that
+// applies it needs no literal.
 constexpr uint32_t kDropOffset = 0x40800000u;
+static_assert(__builtin_bit_cast(uint32_t, 4.0f) == kDropOffset, "the select below writes the offset as 4.0");
 constexpr int32_t kRawBufferWord3 = 0x00020000; // raw 32-bit buffer access (GFX9 family)
 
 template <bool NARROW, bool WZERO, bool WANT_R, bool WIDE, bool DENSE = false>
@@ -977,9 +993,8 @@ __global__ __launch_bounds__(BT) void count_kernel(
     __shared__ BrickLDS<BT> L;
     __shared__ double Csh[9];
     const PairsArgs A{sys, sys_cell_base, nsys, M, nat, cell_start, rec, wbig};
-    const int b = (int)xcd_remap(blockIdx.x, gridDim.x);
     Brick B;
-    brick_locate(A, sys_brick_base, b, BrickShape<BT>::halo, B);
+    brick_locate(A, sys_brick_base, (int)blockIdx.x, BrickShape<BT>::halo, B);
     if (threadIdx.x < 9) Csh[threadIdx.x] = B.sd->C[threadIdx.x];   // visible after brick_load's barrier
     const bool lds = B.fast && brick_load(A, B, L);
     if (!lds) {
