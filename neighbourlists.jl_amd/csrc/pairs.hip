@@ -33,6 +33,15 @@ namespace nlh {
 
 __device__ __forceinline__ int sgpr(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// v with lane `lane` (wave-uniform) set to the wave-uniform x: one
+// v_writelane, no exec-mask or compare work.
+__device__ __forceinline__ int32_t write_lane(int32_t x, int lane, int32_t v)
+{
+    // the lane select in M0: the value's SGPR is the one constant-bus read
+    __asm__("v_writelane_b32 %0, %1, m0" : "+v"(v) : "s"(sgpr(x)), "{m0}"(sgpr(lane)));
+    return v;
+}
+
 __device__ __forceinline__ uint64_t shfl64(uint64_t v, int src)
 {
     const uint32_t lo = __shfl((uint32_t)v, src, kWave), hi = __shfl((uint32_t)(v >> 32), src, kWave);
@@ -834,12 +843,16 @@ __device__ __forceinline__ void test_home_pair(int nch, const double *cx, const 
 // test_home_pair for three full chunks with no shift and 4-byte rows without
 // R: the six distance tests of the pair, then its six emissions (every lane
 // stores, lanes without a hit at the dropped offset), one branch-free block.
+// G: the chunk that holds both self candidates (known for a whole group of
+// home atoms: one mask per atom), kBrickChunks when neither lies in the
+// pass, or -1 when the chunk is found per atom.
+template <int G>
 __device__ __forceinline__ void test_pair_plain(const double *cx, const double *cy, const double *cz,
                                                 const uint64_t *ent, double h0x, double h0y, double h0z, double h1x,
                                                 double h1y, double h1z, double rc2, int ks0, void *__restrict__ row0,
                                                 void *__restrict__ row1, uint32_t cap, int32_t &ca0, int32_t &ca1)
 {
-    const int gs0 = ks0 >> 6, gs1 = (ks0 + 1) >> 6;
+    const int gs0 = G < 0 ? ks0 >> 6 : G, gs1 = G < 0 ? (ks0 + 1) >> 6 : G;
     const uint64_t nself0 = ~(1ull << (ks0 & (kWave - 1))), nself1 = ~(1ull << ((ks0 + 1) & (kWave - 1)));
     uint64_t m0[kBrickChunks], m1[kBrickChunks];
 #pragma unroll
@@ -859,6 +872,27 @@ __device__ __forceinline__ void test_pair_plain(const double *cx, const double *
     }
 }
 
+// The home atoms of a pass through test_pair_plain<G>, two at a time from
+// home atom a (left at the first atom not done).
+template <int G>
+__device__ __forceinline__ void plain_pairs(const double *cx, const double *cy, const double *cz, const uint64_t *ent,
+                                            const LAtom *home, int nh, int32_t hj_l, int kpass, double rc2,
+                                            int64_t atom_base, void *__restrict__ slots, int32_t cap, int32_t &cnt_l,
+                                            int &a)
+{
+    for (; a + 1 < nh; a += 2) {
+        const LAtom h0 = home[a], h1 = home[a + 1];
+        int32_t ca0 = __builtin_amdgcn_readlane(cnt_l, a), ca1 = __builtin_amdgcn_readlane(cnt_l, a + 1);
+        const int64_t srow0 = slot_row(atom_base + __builtin_amdgcn_readlane(hj_l, a), cap);
+        const int64_t srow1 = slot_row(atom_base + __builtin_amdgcn_readlane(hj_l, a + 1), cap);
+        test_pair_plain<G>(cx, cy, cz, ent, h0.x, h0.y, h0.z, h1.x, h1.y, h1.z, rc2, kpass + a,
+                           reinterpret_cast<uint64_t *>(slots) + srow0, reinterpret_cast<uint64_t *>(slots) + srow1,
+                           (uint32_t)cap, ca0, ca1);
+        cnt_l = write_lane(ca0, a, cnt_l);
+        cnt_l = write_lane(ca1, a + 1, cnt_l);
+    }
+}
+
 // One pass of a home cell's atoms over loaded candidate chunks: every home
 // atom's distance tests against the nch chunks (cx, cy, cz, ent: j |
 // pack(s - w_j) << 32, or j | S code << 26 in a 4-byte brick), its hits
@@ -873,7 +907,6 @@ __device__ __forceinline__ void pass_home_atoms(bool narrow, int nch, const doub
                                                 int64_t atom_base, void *__restrict__ slots, double *__restrict__ slotsR,
                                                 int32_t cap, Status *__restrict__ status, int32_t &cnt_l)
 {
-    const int lane = lane_id();
     double sx[kBrickChunks], sy[kBrickChunks], sz[kBrickChunks];
     uint32_t wcur = 0xffffffffu;   // home winding the shifts were computed for
     int32_t wix = 0, wiy = 0, wiz = 0;
@@ -899,17 +932,19 @@ __device__ __forceinline__ void pass_home_atoms(bool narrow, int nch, const doub
         }
         wcur = kPackZero;
         if (DENSE && !WANT_R && !WIDE && nch == kBrickChunks && (csh[0] | csh[1] | csh[2]) == 0) {
-            for (; a + 1 < nh; a += 2) {
-                const LAtom h0 = home[a], h1 = home[a + 1];
-                int32_t ca0 = __builtin_amdgcn_readlane(cnt_l, a), ca1 = __builtin_amdgcn_readlane(cnt_l, a + 1);
-                const int64_t srow0 = slot_row(atom_base + __builtin_amdgcn_readlane(hj_l, a), cap);
-                const int64_t srow1 = slot_row(atom_base + __builtin_amdgcn_readlane(hj_l, a + 1), cap);
-                test_pair_plain(cx, cy, cz, ent, h0.x, h0.y, h0.z, h1.x, h1.y, h1.z, rc2, kpass + a,
-                                reinterpret_cast<uint64_t *>(slots) + srow0, reinterpret_cast<uint64_t *>(slots) + srow1,
-                                (uint32_t)cap, ca0, ca1);
-                cnt_l = __builtin_amdgcn_inverse_ballot_w64(1ull << a) ? ca0 : cnt_l;
-                cnt_l = __builtin_amdgcn_inverse_ballot_w64(2ull << a) ? ca1 : cnt_l;
-            }
+            // the self candidates kpass .. kpass + nh - 1 of the group: in one
+            // chunk (nearly always), in none, or across two
+            const int g0 = kpass >> 6, g1 = (kpass + nh - 1) >> 6;
+            if (g1 < 0 || g0 >= kBrickChunks)
+                plain_pairs<kBrickChunks>(cx, cy, cz, ent, home, nh, hj_l, kpass, rc2, atom_base, slots, cap, cnt_l, a);
+            else if (g0 == g1 && g0 == 0)
+                plain_pairs<0>(cx, cy, cz, ent, home, nh, hj_l, kpass, rc2, atom_base, slots, cap, cnt_l, a);
+            else if (g0 == g1 && g0 == 1)
+                plain_pairs<1>(cx, cy, cz, ent, home, nh, hj_l, kpass, rc2, atom_base, slots, cap, cnt_l, a);
+            else if (g0 == g1 && g0 == 2)
+                plain_pairs<2>(cx, cy, cz, ent, home, nh, hj_l, kpass, rc2, atom_base, slots, cap, cnt_l, a);
+            else
+                plain_pairs<-1>(cx, cy, cz, ent, home, nh, hj_l, kpass, rc2, atom_base, slots, cap, cnt_l, a);
         }
         for (; a + 1 < nh; a += 2) {
             // uniform LDS reads: every lane receives the home atom's coordinates
@@ -928,8 +963,8 @@ __device__ __forceinline__ void pass_home_atoms(bool narrow, int nch, const doub
             test_home_pair<WANT_R, WIDE, DENSE>(nch, cx, cy, cz, sx, sy, sz, csh, ent, h0x, h0y, h0z, h1x, h1y,
                                          h1z, rc2, kpass + a, row0, row1, rrow0, rrow1,
                                          (uint32_t)cap, status, ca0, ca1);
-            cnt_l = __builtin_amdgcn_inverse_ballot_w64(1ull << a) ? ca0 : cnt_l;
-            cnt_l = __builtin_amdgcn_inverse_ballot_w64(2ull << a) ? ca1 : cnt_l;
+            cnt_l = write_lane(ca0, a, cnt_l);
+            cnt_l = write_lane(ca1, a + 1, cnt_l);
         }
     }
     for (; a < nh; ++a) {
@@ -973,7 +1008,7 @@ __device__ __forceinline__ void pass_home_atoms(bool narrow, int nch, const doub
             test_home_atom<false, false, WANT_R, WIDE>(nch, cx, cy, cz, sx, sy, sz, csh, ent, hx, hy, hz,
                                                        rc2, ks, row, rrow, (uint32_t)cap, status, ca, wix,
                                                        wiy, wiz);
-        cnt_l = lane == a ? ca : cnt_l;
+        cnt_l = write_lane(ca, a, cnt_l);
     }
 }
 
