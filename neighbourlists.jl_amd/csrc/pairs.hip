@@ -872,6 +872,26 @@ __device__ __forceinline__ void test_pair_plain(const double *cx, const double *
     }
 }
 
+// One home atom against three full chunks with no shift, 4-byte rows
+// without R (the odd atom of a pass through test_pair_plain).
+__device__ __forceinline__ void test_atom_plain(const double *cx, const double *cy, const double *cz,
+                                                const uint64_t *ent, double hx, double hy, double hz, double rc2,
+                                                int ks, void *__restrict__ row, uint32_t cap, int32_t &ca)
+{
+    const int gs = ks >> 6;
+    const uint64_t nself = ~(1ull << (ks & (kWave - 1)));
+    uint64_t m[kBrickChunks];
+#pragma unroll
+    for (int g = 0; g < kBrickChunks; ++g) {
+        // R = xj - xi (shift zero) ; r^2 = dot(R, R)  (gpu_kernels.jl:89-92)
+        const double Rx = cx[g] - hx, Ry = cy[g] - hy, Rz = cz[g] - hz;
+        m[g] = __builtin_amdgcn_ballot_w64((Rx * Rx + Ry * Ry) + Rz * Rz < rc2) & (g == gs ? nself : ~0ull);
+    }
+#pragma unroll
+    for (int g = 0; g < kBrickChunks; ++g)
+        emit_row<true, true, false, false, true>(row, nullptr, cap, nullptr, m[g], ca, ent[g], 0, 0, 0, 0.0, 0.0, 0.0);
+}
+
 // The home atoms of a pass through test_pair_plain<G>, two at a time from
 // home atom a (left at the first atom not done).
 template <int G>
@@ -918,19 +938,8 @@ __device__ __forceinline__ void pass_home_atoms(bool narrow, int nch, const doub
         // every home atom of a 4-byte brick has w_i = 0: the shifts of
         // the candidates are their own cell shifts
 #pragma unroll
-        for (int g = 0; g < kBrickChunks; ++g) {
-            csh[g] = 0;
-            if (g < nch) {
-                const uint32_t t = (uint32_t)(ent[g] >> 32);
-                csh[g] = __builtin_amdgcn_ballot_w64(t != kPackZero);
-                if (WANT_R || csh[g] != 0) {
-                    int32_t cx_, cy_, cz_;
-                    unpack_wind(t, cx_, cy_, cz_);
-                    cell_shift(Cm, cx_, cy_, cz_, sx[g], sy[g], sz[g]);
-                }
-            }
-        }
-        wcur = kPackZero;
+        for (int g = 0; g < kBrickChunks; ++g)
+            csh[g] = g < nch ? __builtin_amdgcn_ballot_w64((uint32_t)(ent[g] >> 32) != kPackZero) : 0;
         if (DENSE && !WANT_R && !WIDE && nch == kBrickChunks && (csh[0] | csh[1] | csh[2]) == 0) {
             // the self candidates kpass .. kpass + nh - 1 of the group: in one
             // chunk (nearly always), in none, or across two
@@ -945,7 +954,25 @@ __device__ __forceinline__ void pass_home_atoms(bool narrow, int nch, const doub
                 plain_pairs<2>(cx, cy, cz, ent, home, nh, hj_l, kpass, rc2, atom_base, slots, cap, cnt_l, a);
             else
                 plain_pairs<-1>(cx, cy, cz, ent, home, nh, hj_l, kpass, rc2, atom_base, slots, cap, cnt_l, a);
+            if (a < nh) {   // the odd home atom
+                const LAtom hm = home[a];
+                int32_t ca = __builtin_amdgcn_readlane(cnt_l, a);
+                const int64_t srow = slot_row(atom_base + __builtin_amdgcn_readlane(hj_l, a), cap);
+                test_atom_plain(cx, cy, cz, ent, hm.x, hm.y, hm.z, rc2, kpass + a,
+                                reinterpret_cast<uint64_t *>(slots) + srow, (uint32_t)cap, ca);
+                cnt_l = write_lane(ca, a, cnt_l);
+            }
+            return;   // the shifts are never needed
         }
+#pragma unroll
+        for (int g = 0; g < kBrickChunks; ++g) {
+            if (g < nch && (WANT_R || csh[g] != 0)) {
+                int32_t cx_, cy_, cz_;
+                unpack_wind((uint32_t)(ent[g] >> 32), cx_, cy_, cz_);
+                cell_shift(Cm, cx_, cy_, cz_, sx[g], sy[g], sz[g]);
+            }
+        }
+        wcur = kPackZero;
         for (; a + 1 < nh; a += 2) {
             // uniform LDS reads: every lane receives the home atom's coordinates
             const LAtom h0 = home[a], h1 = home[a + 1];
