@@ -451,6 +451,7 @@ struct BrickLDS {
     // candidate of the current pass marking where a non-empty row starts
     int32_t rowdl[Sh::waves][16];
     uint32_t rowflag[Sh::waves][kPassCands / 4];
+    uint16_t owner[Sh::cap];        // halo cell of each LDS atom slot (while loading)
     int32_t total, flag;
     int32_t row8;                   // some shift needs 8-byte slot entries
 };
@@ -502,7 +503,13 @@ __device__ bool brick_load(const PairsArgs &A, const Brick &B, BrickLDS<BT> &L)
 #pragma unroll
     for (int u = 0; u < 1; ++u) {
         const int h = tid + u;
-        if (h < B.nhalo) L.hoff[h] = run;
+        if (h < B.nhalo) {
+            L.hoff[h] = run;
+            // the cell's atom slots name their cell (a brick whose halo
+            // overflows the LDS takes the global-memory path)
+            const int e = min(run + cnt[u], Sh::cap);
+            for (int q = run; q < e; ++q) L.owner[q] = (uint16_t)h;
+        }
         run += cnt[u];
     }
     if (tid == 0) {
@@ -526,14 +533,9 @@ __device__ bool brick_load(const PairsArgs &A, const Brick &B, BrickLDS<BT> &L)
         pg[u] = 0;
         hs[u] = kPackZero;
         if (idx < total) {
-            int lo = 0, hi = B.nhalo - 1;   // largest h with hoff[h] <= idx
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (L.hoff[mid] <= idx) lo = mid;
-                else hi = mid - 1;
-            }
-            pg[u] = L.hstart[lo] + (idx - L.hoff[lo]);
-            hs[u] = L.hshift[lo];
+            const int h = L.owner[idx];
+            pg[u] = L.hstart[h] + (idx - L.hoff[h]);
+            hs[u] = L.hshift[h];
         }
     }
     Rec r[Sh::unroll];
@@ -893,8 +895,9 @@ __device__ __forceinline__ void test_atom_plain(const double *cx, const double *
 }
 
 // The home atoms of a pass through test_pair_plain<G>, two at a time from
-// home atom a (left at the first atom not done).
-template <int G>
+// home atom a (left at the first atom not done).  FRESH: the group's first
+// pass, every running count still zero.
+template <int G, bool FRESH = false>
 __device__ __forceinline__ void plain_pairs(const double *cx, const double *cy, const double *cz, const uint64_t *ent,
                                             const LAtom *home, int nh, int32_t hj_l, int kpass, double rc2,
                                             int64_t atom_base, void *__restrict__ slots, int32_t cap, int32_t &cnt_l,
@@ -902,7 +905,8 @@ __device__ __forceinline__ void plain_pairs(const double *cx, const double *cy, 
 {
     for (; a + 1 < nh; a += 2) {
         const LAtom h0 = home[a], h1 = home[a + 1];
-        int32_t ca0 = __builtin_amdgcn_readlane(cnt_l, a), ca1 = __builtin_amdgcn_readlane(cnt_l, a + 1);
+        int32_t ca0 = FRESH ? 0 : __builtin_amdgcn_readlane(cnt_l, a);
+        int32_t ca1 = FRESH ? 0 : __builtin_amdgcn_readlane(cnt_l, a + 1);
         const int64_t srow0 = slot_row(atom_base + __builtin_amdgcn_readlane(hj_l, a), cap);
         const int64_t srow1 = slot_row(atom_base + __builtin_amdgcn_readlane(hj_l, a + 1), cap);
         test_pair_plain<G>(cx, cy, cz, ent, h0.x, h0.y, h0.z, h1.x, h1.y, h1.z, rc2, kpass + a,
@@ -919,9 +923,10 @@ __device__ __forceinline__ void plain_pairs(const double *cx, const double *cy, 
 // appended to its slot row at its running count (lane a of cnt_l for home
 // atom a).  home: the nh home atoms in LDS; ht_l, hj_l: lane a holds
 // pack(-w_i) and j of home atom a; kpass: the self candidate of home atom 0
-// in this pass (home atom a's is kpass + a); Cm: the cell matrix.
+// in this pass (home atom a's is kpass + a); Cm: the cell matrix; fresh:
+// the first pass of the home atoms (every running count zero).
 template <bool WANT_R, bool WIDE, bool DENSE>
-__device__ __forceinline__ void pass_home_atoms(bool narrow, int nch, const double *cx, const double *cy,
+__device__ __forceinline__ void pass_home_atoms(bool narrow, bool fresh, int nch, const double *cx, const double *cy,
                                                 const double *cz, const uint64_t *ent, const LAtom *home, int nh,
                                                 uint32_t ht_l, int32_t hj_l, int kpass, const double *Cm, double rc2,
                                                 int64_t atom_base, void *__restrict__ slots, double *__restrict__ slotsR,
@@ -946,6 +951,12 @@ __device__ __forceinline__ void pass_home_atoms(bool narrow, int nch, const doub
             const int g0 = kpass >> 6, g1 = (kpass + nh - 1) >> 6;
             if (g1 < 0 || g0 >= kBrickChunks)
                 plain_pairs<kBrickChunks>(cx, cy, cz, ent, home, nh, hj_l, kpass, rc2, atom_base, slots, cap, cnt_l, a);
+            else if (g0 == g1 && g0 == 0 && fresh)
+                plain_pairs<0, true>(cx, cy, cz, ent, home, nh, hj_l, kpass, rc2, atom_base, slots, cap, cnt_l, a);
+            else if (g0 == g1 && g0 == 1 && fresh)
+                plain_pairs<1, true>(cx, cy, cz, ent, home, nh, hj_l, kpass, rc2, atom_base, slots, cap, cnt_l, a);
+            else if (g0 == g1 && g0 == 2 && fresh)
+                plain_pairs<2, true>(cx, cy, cz, ent, home, nh, hj_l, kpass, rc2, atom_base, slots, cap, cnt_l, a);
             else if (g0 == g1 && g0 == 0)
                 plain_pairs<0>(cx, cy, cz, ent, home, nh, hj_l, kpass, rc2, atom_base, slots, cap, cnt_l, a);
             else if (g0 == g1 && g0 == 1)
@@ -1106,8 +1117,9 @@ __global__ __launch_bounds__(BT) void count_kernel(
                     cz[g] = c.z;
                     ent[g] = (uint64_t)((uint32_t)c.j & jmask) | ((uint64_t)c.t << 32);
                 }
-                pass_home_atoms<WANT_R, WIDE, true>(narrow, nch, cx, cy, cz, ent, L.a + hb, nh, ht_l, hj_l, kself - k0, Csh,
-                                              rc2, sd.atom_base, slots, slotsR, cap, status, cnt_l);
+                pass_home_atoms<WANT_R, WIDE, true>(narrow, k0 == 0, nch, cx, cy, cz, ent, L.a + hb, nh, ht_l, hj_l,
+                                                    kself - k0, Csh, rc2, sd.atom_base, slots, slotsR, cap, status,
+                                                    cnt_l);
             }
             if (lane < nh) pair_count[sd.atom_base + hj_l] = cnt_l | (narrow ? 0 : kRow8Bit);
         }
@@ -1448,7 +1460,7 @@ __global__ __launch_bounds__(kSmallThreads) void small_kernel(SmallArgs A)
                         const uint32_t lo = narrow ? ((uint32_t)c.j | code << 26) : (uint32_t)c.j;
                         ent[g] = (uint64_t)lo | ((uint64_t)t << 32);
                     }
-                    pass_home_atoms<WANT_R, false, false>(narrow, nch, cx, cy, cz, ent, L.a + hb, nh, ht_l, hj_l,
+                    pass_home_atoms<WANT_R, false, false>(narrow, false, nch, cx, cy, cz, ent, L.a + hb, nh, ht_l, hj_l,
                                                    self0 + hg - k0, sd.C, sd.rc2, sd.atom_base, A.slots, A.slotsR,
                                                    A.cap, A.status, cnt_l);
                 }
