@@ -33,6 +33,14 @@ namespace nlh {
 
 __device__ __forceinline__ int sgpr(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// The next work item of a wave from a workgroup counter (wave-uniform).
+__device__ __forceinline__ int take_next(int32_t *ctr)
+{
+    int t = 0;
+    if (lane_id() == 0) t = atomicAdd(ctr, 1);
+    return __builtin_amdgcn_readfirstlane(t);
+}
+
 // v with lane `lane` (wave-uniform) set to the wave-uniform x: one
 // v_writelane, no exec-mask or compare work.
 __device__ __forceinline__ int32_t write_lane(int32_t x, int lane, int32_t v)
@@ -452,6 +460,7 @@ struct BrickLDS {
     int32_t rowdl[Sh::waves][16];
     uint32_t rowflag[Sh::waves][kPassCands / 4];
     uint16_t owner[Sh::cap];        // halo cell of each LDS atom slot (while loading)
+    int32_t next_cell;              // the next home cell a wave takes
     int32_t total, flag;
     int32_t row8;                   // some shift needs 8-byte slot entries
 };
@@ -515,6 +524,7 @@ __device__ bool brick_load(const PairsArgs &A, const Brick &B, BrickLDS<BT> &L)
     if (tid == 0) {
         L.hoff[B.nhalo] = (int32_t)tot;
         L.total = (int32_t)tot;
+        L.next_cell = Sh::waves;
         L.flag = 0;
         L.row8 = sd.nat > kNarrowMaxAtoms ? 1 : 0;
         L.a[Sh::cap].x = L.a[Sh::cap].y = L.a[Sh::cap].z = kFar;
@@ -1085,7 +1095,9 @@ __global__ __launch_bounds__(BT) void count_kernel(
     // L.j holds j | S code << 26 whenever every j fits in 26 bits
     const uint32_t hjmask = sd.nat <= kNarrowMaxAtoms ? (uint32_t)(kNarrowMaxAtoms - 1) : 0xffffffffu;
     const uint32_t jmask = narrow ? 0xffffffffu : hjmask;   // entry low word: j | code << 26, or j
-    for (int hc = wv; hc < B.nhome; hc += BrickShape<BT>::waves) {
+    // home cells: one per wave, then the next one not taken (a wave that
+    // ends its cell early takes the next at once)
+    for (int hc = wv; hc < B.nhome; hc = take_next(&L.next_cell)) {
         const HomeRows R = brick_home(B, L, hc);
         bool rne;
         const int rq = row_publish(R, L, wv, rne);
@@ -1216,14 +1228,6 @@ __device__ __forceinline__ void wrap_small(int32_t a, int32_t n, float rn, bool 
     const int32_t q = (int32_t)floorf(((float)a - 0.5f) * rn);
     w = a - q * n;
     s = q;
-}
-
-// The next work item of a wave from a workgroup counter (wave-uniform).
-__device__ __forceinline__ int take_next(int32_t *ctr)
-{
-    int t = 0;
-    if (lane_id() == 0) t = atomicAdd(ctr, 1);
-    return __builtin_amdgcn_readfirstlane(t);
 }
 
 template <bool WANT_R, typename TI>
