@@ -511,12 +511,13 @@ void assign_bricks(nlh_handle *h)
     h->brick_base_host.resize(nsys + 1);
     for (int s = 0; s < nsys; ++s) {
         nlh::SysDev &sd = h->sys_host[s];
-        // a single system splits into at least two rounds of bricks over the
-        // device: four 256-thread bricks (36 KB of LDS each) or two
-        // 512-thread bricks fit a CU
+        // a single system takes the largest brick that still fills the
+        // device once: four 256-thread bricks (39 KB of LDS each) or two
+        // 512-thread bricks fit a CU, and the largest-first launch order
+        // fills the end of the launch with the short partial bricks
         // the LDS of a BT-thread brick holds 4 BT halo atoms and BT halo cells
         const int bt = h->count_block();
-        const int64_t min_bricks = nsys == 1 ? 2 * (int64_t)h->cus * (bt == 256 ? 4 : 2) : 0;
+        const int64_t min_bricks = nsys == 1 ? (int64_t)h->cus * (bt == 256 ? 4 : 2) : 0;
         choose_brick(h->geoms[s], sd.nat, std::min(h->tune.brick_atoms, (int32_t)(kHaloFill * 4 * bt)), bt,
                      sd.brick, min_bricks);
         sd.brick_base = (int32_t)bricks;
@@ -1458,7 +1459,7 @@ void neighbour_list_noalloc(hipStream_t st, const double *X, int64_t nat, const 
     P.sd = make_sys(g, nat, 0, 0, 0);
     const int cus = g_scratch_cus[dev];
     choose_brick(g, nat, std::min<int32_t>(920, (int32_t)(kHaloFill * 4 * 256)), 256, P.sd.brick,
-                 2 * (int64_t)cus * 4);
+                 (int64_t)cus * 4);
     P.sd.brick_base = 0;
     P.nbricks = brick_classes(P.sd);
     P.atom_base[0] = 0;
