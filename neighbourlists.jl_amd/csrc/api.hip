@@ -197,8 +197,8 @@ struct nlh_handle {
     nlh_tuning tune = {920, 256, 256, 0, 0, 0};
     int cus = 256;   // compute units of the device
 
-    // count-pass workgroup: 4 waves per brick unless tuned (four 36 KB
-    // bricks share a CU; 8-wave bricks of 72 KB fit two)
+    // count-pass workgroup: 4 waves per brick unless tuned (four 39 KB
+    // bricks share a CU; 8-wave bricks of 76 KB fit two)
     int count_block() const { return tune.count_block ? tune.count_block : 256; }
 
     nlh::PipelineBufs bufs() const
