@@ -3,24 +3,27 @@
 // (src/gpu_kernels.jl:132-180) around the inner loop _for_each_neighbor_pair
 // (src/gpu_kernels.jl:58-101).
 //
-// count_kernel -- work unit: a brick of up to 4x4x4 home cells per
-// 256-thread workgroup.  The brick and its one-cell halo are staged in LDS in
-// one load phase (every global load of the phase in flight together), then
-// each wave takes home cells and tests candidates from LDS only:
+// count_kernel -- work unit: a brick of up to 4x4x4 home cells per 256-thread
+// (or 512-thread) workgroup, the bricks launched largest first.  The brick
+// and its one-cell halo are staged in LDS in one load phase (every global
+// load of the phase in flight together), then the waves take home cells from
+// a workgroup counter and test candidates from LDS only:
 //   - the candidates of a home cell are its 27 neighbour cells flattened in
 //     the reference order (dz, dy, dx; gpu_kernels.jl:69-71) -- nine
 //     contiguous LDS rows -- cut into 64-candidate chunks, one per lane;
-//   - the home atoms stream through the wave as uniform values; each
-//     (home atom, chunk) test is one ballot, and the hit lanes append their
-//     (j, S) to the home atom's slot at its running count, so every slot
-//     holds the atom's neighbours in exactly the reference traversal order.
+//   - the home atoms stream through the wave as uniform values, two at a
+//     time; each (home atom, chunk) test is one ballot, and the hit lanes
+//     append their (j, S) to the home atom's slot row at its running count,
+//     so every row holds the atom's neighbours in exactly the reference
+//     traversal order.
 // Bricks whose neighbour reach exceeds one cell, whose halo does not fit
 // LDS, or whose windings do not pack take a per-wave path that reads global
 // memory (one home cell per wave) with the same enumeration and output.
 //
-// fill_kernel -- atoms in their original order, one lane per pair: each wave
-// copies the slots of 64 consecutive atoms into one contiguous range of the
-// PairList arrays (i, j, S and optionally R), so its stores are linear.
+// fill_kernel -- one wave per tile of consecutive output pairs, one lane per
+// pair: the tile's first atom comes from the pair scan's tile table, and
+// the lanes copy the slot-row entries of the atoms the tile covers into the
+// PairList arrays (i, j, S and optionally R), so every store is linear.
 #include <algorithm>
 
 #include "device_util.h"
