@@ -211,6 +211,30 @@ def test_bricks_medium(pbc):
     _check(s.X, 5.0, s.cell, pbc)
 
 
+@pytest.mark.parametrize("brick_atoms,count_block", [(200, 256), (460, 256), (700, 256), (920, 256), (1840, 512)])
+def test_brick_launch_order(brick_atoms, count_block):
+    """Bricks launch largest shape class first, each class in row-major order
+    (count pass).  A batch of boxes of 7 x 9 x 11 and 13 x 6 x 5 cells, whose
+    axes end in partial bricks of different sizes, so that the classes (full,
+    partial along one, two or three axes) differ in size and order, through
+    the batched build (no minimum brick count: each budget keeps its largest
+    brick, 1 x 1 x 1 to 4 x 4 x 4); every home cell is counted exactly once."""
+    from nlhip.batch import BatchBuilder
+    rng = np.random.default_rng(83)
+    systems = []
+    for dims, pbc in [((7, 9, 11), (True, True, True)), ((13, 6, 5), (False, True, False)),
+                      ((7, 9, 11), (True, False, True))]:
+        C = np.diag([d * (5.0 + 0.05 * k) for k, d in enumerate(dims)])
+        X = rng.random((int(0.05 * np.prod(np.diag(C))), 3)) @ C
+        systems.append(workloads.System(X, C, pbc, 5.0))
+    print(f"brick order brick_atoms={brick_atoms} count_block={count_block}", flush=True)
+    bb = BatchBuilder()
+    bb.handle.set_tuning(brick_atoms, 256, 256, count_block, 0, 1)
+    res = bb.run(systems, want_R=True)
+    for k, sysm in enumerate(systems):
+        _same(res.pairlist(k), oracle.neighbour_list(sysm.X, sysm.cutoff, sysm.cell, sysm.pbc, want_R=True))
+
+
 def test_slot_rows_overflow_and_regrow():
     # 80% of the atoms packed into one eighth of the box: the slot rows sized
     # from the mean density (about 8 neighbours, 32 slots) are too short for
