@@ -110,3 +110,24 @@ def test_errors_and_edges():
     # and the scratch still serves the next call
     print("no-alloc: after the refusals", flush=True)
     _same_as_handle_path(s.X, s.cell, 5.0, s.pbc)
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_random_geometries(seed):
+    """The random geometries of the parity sweep through the entry point
+    that allocates nothing, one call after another on the shared scratch,
+    against the oracle."""
+    import oracle
+    from test_gpu_sweep import _random_system
+
+    s, int_type, want_R = _random_system(seed, 20000, 2.0)
+    ti = 4 if int_type == np.int32 else 8
+    print(f"no-alloc sweep seed={seed} N={s.X.shape[0]} pbc={s.pbc} ti={ti} R={want_R}", flush=True)
+    ref = oracle.neighbour_list(s.X, s.cutoff, s.cell, s.pbc, ti_bits=8 * ti, want_R=want_R, nthreads=8)
+    st, n, out = _noalloc(s.X, s.cell, s.cutoff, s.pbc, want_R=want_R, ti=ti, cap=ref.npairs)
+    assert st == _lib.NLH_OK, _lib.lib().nlh_last_error()
+    assert n == ref.npairs
+    for k in ("i", "j", "S", "first"):
+        assert np.array_equal(out[k], getattr(ref, k)), k
+    if want_R:
+        assert np.array_equal(out["R"].view(np.int64), ref.R.view(np.int64))
