@@ -9,7 +9,8 @@ pair list (i, j, S, first, R) are compared element for element with the
 oracle, the C restatement of the reference's sort-based path
 (cell_list.jl:632-748, gpu_kernels.jl:20-364).  The batch cases put 80 such
 small systems through one batched build (the fused small-system kernel when
-every system fits its limits) and check every system.
+every system fits its limits) and check every system; the lazy cases query
+every atom's neighbours from the cell list.
 """
 import numpy as np
 import pytest
@@ -83,3 +84,20 @@ def test_random_geometry_batch(seed):
         assert np.array_equal(pl.first, ref.first), k
         assert np.array_equal(pl.i, ref.i) and np.array_equal(pl.j, ref.j) and np.array_equal(pl.S, ref.S), k
         assert np.array_equal(np.asarray(pl.R).view(np.int64), ref.R.view(np.int64)), k
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_random_geometry_lazy(seed):
+    """The host lazy API (neighbours over a range of atoms, cell_list.jl:779-833)
+    on the cell list of each draw: every atom's j, S and R equal the oracle's
+    slice of the materialised list."""
+    s, int_type, _ = _random_system(100 + seed, 20000, 2.0)
+    nat = s.X.shape[0]
+    print(f"sweep lazy seed={seed} N={nat} pbc={s.pbc}", flush=True)
+    ti = 32 if int_type == np.int32 else 64
+    ref = oracle.neighbour_list(s.X, s.cutoff, s.cell, s.pbc, ti_bits=ti, want_R=True, nthreads=8)
+    cl = nlhip.build_cell_list(s.X, s.cutoff, s.cell, s.pbc, int_type=int_type)
+    offs, js, Rs, Ss = nlhip.neighbours_range(cl, 1, nat)
+    assert np.array_equal(offs + 1, ref.first)
+    assert np.array_equal(js, ref.j) and np.array_equal(Ss, ref.S)
+    assert np.array_equal(np.asarray(Rs).view(np.int64), ref.R.view(np.int64))
