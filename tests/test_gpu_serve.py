@@ -135,3 +135,35 @@ def test_served_latency_covers_the_fill():
     assert all(c.end >= c.returned >= c.start for c in done)
     s = serve.summarize(done)
     assert s["p50_ms"] >= s["p50_return_ms"] and s["p99_ms"] >= s["p99_return_ms"]
+
+
+def test_random_geometry_stream():
+    """Requests drawn by the parity sweep's generator (random triclinic cells,
+    pbc patterns, cutoffs up to 2.5 face heights, atoms outside the cell),
+    a third of them rebuilds of an earlier system with new positions, served
+    mixed (batched small systems, residents for the large ones); every
+    completion is checked against the oracle."""
+    from test_gpu_sweep import _random_system
+
+    u = workloads.uniform(4242, 48)
+    reqs, base = [], []
+    for k in range(48):
+        if base and u[k] < 0.33:
+            sid = int(workloads.uniform(4343 + k, 1)[0] * len(base))
+            b = base[sid]
+            X = b.X + 0.05 * workloads.normal(4444 + k, b.X.size).reshape(b.X.shape)
+            reqs.append(serve.Request(k, sid, X, b.cell, b.pbc, b.cutoff, True))
+            continue
+        s, _, _ = _random_system(300 + k, 20000, 2.0)
+        reqs.append(serve.Request(k, len(base), s.X, s.cell, s.pbc, s.cutoff, False))
+        base.append(s)
+    assert any(r.rebuild for r in reqs) and any(r.nat >= 2048 for r in reqs)
+    srv = serve.Server(workers=3, small_atoms=2048, batch_atoms=20000, keep_results=True)
+    done = srv.serve(reqs, realtime=False)
+    srv.close()
+    assert [c.rid for c in done] == [r.rid for r in reqs]
+    for c, r in zip(done, reqs):
+        print(f"request {r.rid} nat={r.nat} pbc={r.pbc} rebuild={r.rebuild}", flush=True)
+        ref = oracle.neighbour_list(r.X, r.cutoff, r.cell, r.pbc, want_R=False, nthreads=8)
+        assert c.npairs == ref.npairs
+        _same(c.result, ref)
