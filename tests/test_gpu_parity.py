@@ -141,6 +141,36 @@ def test_far_windings():
     _check(Xp, 0.3 * L, s.cell, (True, True, True))
 
 
+@pytest.mark.parametrize("bad", [np.nan, np.inf, -np.inf, 1.0e300])
+def test_non_finite_positions_are_refused(bad):
+    """A position whose cell index does not fit (non-finite, or beyond 1e9
+    cells; the reference's unsafe_trunc is undefined there, cell_list.jl:66-74)
+    makes the build fail with NLH_EINVAL, on the single-system path and on the
+    fused small-system build of a batch; the next builds are unaffected."""
+    from nlhip.batch import BatchBuilder
+
+    s = workloads.random_cubic(3000, seed=61)
+    X = s.X.copy()
+    X[1234, 1] = bad
+    print(f"bad position {bad}: single system", flush=True)
+    with pytest.raises(nlhip.NLHInvalidError):
+        nlhip.neighbour_list(X, 5.0, s.cell, s.pbc)
+    _check(s.X, 5.0, s.cell, s.pbc)
+    systems = workloads.batch_systems(80, seed=62, nmax=600)
+    bad_systems = list(systems)
+    Xb = systems[17].X.copy()
+    Xb[0, 2] = bad
+    bad_systems[17] = workloads.System(Xb, systems[17].cell, systems[17].pbc, systems[17].cutoff)
+    print(f"bad position {bad}: batch", flush=True)
+    bb = BatchBuilder()
+    with pytest.raises(nlhip.NLHInvalidError):
+        bb.run(bad_systems)
+    res = bb.run(systems, want_R=True)
+    for k in (0, 17, 79):
+        sysm = systems[k]
+        _same(res.pairlist(k), oracle.neighbour_list(sysm.X, sysm.cutoff, sysm.cell, sysm.pbc, want_R=True))
+
+
 def test_sorted_cell_list_fields():
     s = workloads.random_cubic(500, seed=51)
     cl = nlhip.build_cell_list(s.X, 5.0, s.cell, s.pbc)
