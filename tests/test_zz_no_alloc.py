@@ -103,6 +103,11 @@ def test_errors_and_edges():
     # no atoms: first = [1]
     st, n, out = _noalloc(np.zeros((0, 3)), s.cell, 5.0, s.pbc, want_R=False, cap=0)
     assert st == _lib.NLH_OK and n == 0 and list(out["first"]) == [1]
+    # a non-finite position: NLH_EINVAL once the binning has seen it
+    Xn = s.X.copy()
+    Xn[4321, 0] = np.nan
+    st, n, _ = _noalloc(Xn, s.cell, 5.0, s.pbc, want_R=False, cap=ref.npairs())
+    assert st == _lib.NLH_EINVAL and b"finite" in L.nlh_last_error()
     # beyond the static scratch: a clear error before any launch
     big = workloads.random_cubic(1_000_000, seed=177)
     st, n, _ = _noalloc(big.X, big.cell, 5.0, big.pbc, want_R=False, cap=16)
