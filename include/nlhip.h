@@ -35,7 +35,11 @@ extern "C" {
 
 typedef enum {
     NLH_OK = 0,
-    NLH_EINVAL = 1,     /* invalid argument (bad cell, cutoff, positions) */
+    NLH_EINVAL = 1,     /* invalid argument (bad cell, cutoff, positions:
+                           a coordinate whose cell index is not finite or
+                           lies beyond 1e9 cells; the reference's
+                           unsafe_trunc is undefined there,
+                           src/cell_list.jl:66-74)                        */
     NLH_EOVERFLOW = 2,  /* cell count or pair count exceeds the index type */
     NLH_ENOMEM = 3,     /* device allocation failed / budget exceeded      */
     NLH_EDEVICE = 4,    /* HIP runtime error                               */
