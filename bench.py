@@ -316,13 +316,13 @@ def _cpu_baseline(s) -> dict:
     reference's benchmark does (scripts/benchmark.jl:124-151: minimum of at
     least 5 samples after a warm-up), at N threads and at one thread."""
     nproc, ncpu, nt = _cpu_share()
-    P, ts = _time_cpu(s, nt, 5)
-    _, t1 = _time_cpu(s, 1, 5)
+    P, ts = _time_cpu(s, nt, 20)   # about 15 s of CPU time at 16 threads
+    _, t1 = _time_cpu(s, 1, 10)    # about 5 s
     return {"value": round(P / min(ts), 1), "unit": "pairs/s", "cores": nt, "kind": "port",
             "median_value": round(P / statistics.median(ts), 1),
             "value_1_thread": round(P / min(t1), 1), "median_value_1_thread": round(P / statistics.median(t1), 1),
             "nproc": nproc, "cpu_count": ncpu, "cpu_model": _cpu_model(),
-            "sample": f"the 100k-atom cube, 1 warm-up then 5 full builds at {nt} threads and 5 at one thread "
+            "sample": f"the 100k-atom cube, 1 warm-up then 20 full builds at {nt} threads and 10 at one thread "
                       f"(min reported, median beside it); C restatement of the reference CPU sort-based path "
                       f"(serial binning, sort and offsets; count and fill over the threads); threads = nproc "
                       f"({nproc}) capped at 16, one GPU's host-CPU share"}
