@@ -6,7 +6,8 @@
   655-659, gpu_kernels.jl:315-317) equals the oracle bit for bit;
 * argument errors map to the reference's exceptions;
 * the seeded generators reproduce published SplitMix64 values;
-* without a device the product path raises instead of computing on the CPU.
+* without a device the product path raises instead of computing on the CPU;
+* bench.py refuses more than one GPU.
 """
 import ctypes as C
 import os
@@ -209,3 +210,15 @@ def test_julia_shim_binds_declared_symbols():
         m = re.match(r"NTuple\{(\d+),(\w+)\}", typ)
         total += int(m.group(1)) * sizes[m.group(2)] if m else sizes[typ.strip()]
     assert total == C.sizeof(_lib.Geometry) == 248
+
+
+def test_bench_measures_one_gpu_only():
+    # the bench's contract: more than one GPU is refused with a message,
+    # before anything touches a device
+    import subprocess
+    import sys
+
+    r = subprocess.run([sys.executable, os.path.join(REPO, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr and r.stdout == ""
